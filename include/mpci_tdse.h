@@ -1,0 +1,153 @@
+/*
+ * mpci_tdse.h -- C ABI of the MI355X (gfx950) tdse propagation engine.
+ *
+ * Drop-in replacement for the propagation path of MP-CITDSE's `tdse`
+ * program.  Plain C types only: host pointers and sizes, no torch or HIP
+ * types.  Every function returns MPCI_OK (0) or a negative MPCI_ERR_* code
+ * and never throws; mpci_last_error() returns the message of the last
+ * failure on the calling thread.  All arithmetic is FP64.
+ *
+ * Objects:
+ *   mpci_system  the device-resident block-tridiagonal operator: energies
+ *                E_L and dipole blocks D_L (what tdrd::readEnergies and
+ *                tdrd::readDipoles load, src/tdse/td_read.cpp:50-120, and
+ *                what MatVecV / MatVecL hold, src/tdse/tdse.cpp:5-12,76-83).
+ *   mpci_prop    P independent pulses propagated on one system, each with
+ *                its own coefficient vector, field and step control.
+ *
+ * State vectors are 2N doubles, interleaved (Re c_k, Im c_k), exactly the
+ * reference's `ct` (src/tdse/tdse_main.cpp:148-150).  Host buffers stay
+ * owned by the caller and are copied; no caller pointer is retained after a
+ * call returns.  A mpci_prop owns one HIP stream; it is not thread-safe,
+ * several may coexist on one device.  Calls block until their work is done.
+ */
+#ifndef MPCI_TDSE_H_
+#define MPCI_TDSE_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPCI_OK 0
+#define MPCI_ERR_ARG (-1)      /* invalid argument */
+#define MPCI_ERR_HIP (-2)      /* HIP runtime error (no GPU, OOM, ...) */
+#define MPCI_ERR_STATE (-3)    /* object not in a state for this call */
+#define MPCI_ERR_LIMIT (-4)    /* attempt limit reached (adaptive) */
+#define MPCI_ERR_NOTSUP (-5)   /* combination not supported */
+
+#define MPCI_PULSE_SINE2 0     /* sin^2 envelope, pulse.cpp:41-56,69-97 */
+#define MPCI_PULSE_GAUSS 1     /* gaussian envelope, pulse.cpp:10-39 */
+#define MPCI_PULSE_PWL 2       /* extension: piecewise-linear envelope */
+
+/* pulse::params (src/tdse/include/pulse.hpp:20-26) plus, for
+ * MPCI_PULSE_PWL, F(t) = scale * env(t) * sin(w t + cepd) on [0, t_end]
+ * with env interpolating n_env samples placed uniformly on [0, t_end]. */
+typedef struct mpci_pulse {
+  int kind;
+  double Eo, w, cepd, a, b;
+  double t_end, scale;
+  int n_env;
+  const double *env;
+} mpci_pulse;
+
+typedef struct mpci_stats {
+  int accepted;   /* accepted steps */
+  int rejected;   /* rejected attempts */
+  int attempts;
+  int status;     /* 0 ok, MPCI_ERR_LIMIT if max_attempts was reached */
+  double t;       /* final time */
+  double h;       /* next proposed step */
+} mpci_stats;
+
+typedef struct mpci_system mpci_system;
+typedef struct mpci_prop mpci_prop;
+
+const char *mpci_last_error(void);
+/* Number of visible GPUs (0 without a GPU). */
+int mpci_device_count(int *n);
+
+/* ---- pulses: src/tdse/pulse.cpp ------------------------------------- */
+/* pulse::toAU (pulse.cpp:3-6): W/cm^2, eV -> atomic units. */
+void mpci_to_au(double Io_W, double w_eV, double *Io_au, double *w_au);
+/* pulse::period (pulse.cpp:8). */
+double mpci_period(double w, int cycles);
+/* pulse::sineASetup (gauge 'v') / sineESetup (gauge 'l'), pulse.cpp:41-75. */
+int mpci_pulse_sine(double Io, double w, double cepd, int cycles, char gauge,
+                    mpci_pulse *out);
+/* pulse::gaussASetup / gaussESetup, pulse.cpp:10-32. */
+int mpci_pulse_gauss(double Io, double w, double tau, int cycles, double cepd,
+                     char gauge, mpci_pulse *out);
+/* Host evaluation: sineAA / sineEE / gauss (pulse.cpp:34-39,52-56,93-97). */
+double mpci_pulse_eval(const mpci_pulse *p, double t);
+
+/* ---- system ---------------------------------------------------------- */
+/* Replaces the MatVecV / MatVecL set-up of tdse::propV / propL
+ * (src/tdse/tdse.cpp:160-169, 243-252): nb = L_max + 1 blocks of sizes
+ * s[L]; E[L] has s[L] energies; D[L] (L < nb - 1) is the s[L] x s[L+1]
+ * row-major dipole block read by tdrd::readDipoles (td_read.cpp:88-120).
+ * gauge 'v' = MatVecV (velocity), 'l' = MatVecL (length). */
+int mpci_system_create(int nb, const int *s, char gauge,
+                       const double *const *E, const double *const *D,
+                       mpci_system **out);
+int mpci_system_destroy(mpci_system *sys);
+/* N = sum of s; bytes = device memory held by the system. */
+int mpci_system_info(const mpci_system *sys, int *N, long long *bytes);
+
+/* ---- propagator ------------------------------------------------------ */
+int mpci_prop_create(mpci_system *sys, int P, mpci_prop **out);
+int mpci_prop_destroy(mpci_prop *prop);
+/* ct: 2N doubles for pulse p (tdse_main.cpp:148-160 initial state). */
+int mpci_prop_set_state(mpci_prop *prop, int p, const double *ct);
+int mpci_prop_get_state(mpci_prop *prop, int p, double *ct);
+int mpci_prop_set_pulse(mpci_prop *prop, int p, const mpci_pulse *pulse);
+
+/* One RHS evaluation per pulse at frozen fields F[p]: dxdt = f(x) for the
+ * 2N-double vectors x[p] (MatVecV/MatVecL::operator(), tdse.cpp:14,85).
+ * x and dxdt hold P consecutive 2N-double vectors.  Overwrites the state. */
+int mpci_prop_rhs(mpci_prop *prop, const double *F, const double *x,
+                  double *dxdt);
+
+/* Reference mode (tdse::propV / propL loop, tdse.cpp:186-204): `steps`
+ * fixed RKF78 steps of size dt with the field frozen per step at
+ * F[p * steps + st] (the caller evaluates field(pars, t + dt)), each step
+ * followed by division by the 2-norm.  pop (P x steps, may be NULL) gets
+ * |c_track|^2 after each step.  t0 only labels the steps. */
+int mpci_prop_fixed(mpci_prop *prop, double t0, double dt, int steps,
+                    const double *F, int track, double *pop);
+
+/* Adaptive mode (extension; the reference has none): embedded-error
+ * controlled RKF78 from t0[p] to t1[p] (last step clipped) with initial
+ * step h0[p], field evaluated from the pulse at the stage times, error
+ * ratio max_i |err_i| / (atol + rtol (|x_i| + h |k1_i|)), renormalised
+ * after each accepted step.  stats: P entries.  log (may be NULL): per
+ * pulse log_cap entries of (t, h, |c_track|^2) for the accepted steps. */
+int mpci_prop_adaptive(mpci_prop *prop, const double *t0, const double *t1,
+                       const double *h0, double atol, double rtol,
+                       int max_attempts, int track, mpci_stats *stats,
+                       double *log, int log_cap);
+
+/* Replays the adaptive map over n given steps hs[p * n + k] from t0[p]
+ * (no controller): used for frozen-schedule finite differences. */
+int mpci_prop_schedule(mpci_prop *prop, const double *t0, int n,
+                       const double *hs);
+
+/* On-device weighted populations: out[o * P + p] = sum_k W[o*N + k]
+ * |c_{k,p}|^2 (ground population, yield and excitation masks of
+ * pes::genPES2e, src/pes/pes.cpp:199-213; all-ones for the norm). */
+int mpci_prop_populations(mpci_prop *prop, int nobs, const double *W,
+                          double *out);
+
+/* Launch the 13-stage step as one captured hipGraph (1, default) or as
+ * individual launches (0). */
+int mpci_prop_set_graph(mpci_prop *prop, int enable);
+/* Instrumentation: when on, steps are launched individually with HIP
+ * events around every coupling kernel, accumulated into
+ * mpci_prop_kernel_time.  Off by default. */
+int mpci_prop_set_instrument(mpci_prop *prop, int enable);
+int mpci_prop_kernel_time(mpci_prop *prop, double *total_ms,
+                          long long *launches, long long *bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPCI_TDSE_H_ */

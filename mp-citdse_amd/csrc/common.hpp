@@ -1,0 +1,104 @@
+// Shared definitions of the MI355X tdse engine (host and device).
+//
+// The engine propagates dc/dt = -i[diag(E) + F(t) K] c for P pulses that
+// share one block-tridiagonal system (blocks L = 0..nb-1, couplings only
+// between L and L+1), the operator of MatVecV / MatVecL in the reference
+// (src/tdse/tdse.cpp:14-143).  All arithmetic is FP64.
+//
+// State layout in HBM: complex coefficient (row i, pulse p) at double2
+// index i * P + p, i.e. for P = 1 exactly the reference's interleaved
+// (Re, Im) vector (tdse_main.cpp:148).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mpci {
+
+constexpr int kMaxBlocks = 32;   // L_max <= 31
+constexpr int kStages = 13;      // Fehlberg 7(8)
+
+// ---- fused GEMV tile geometry (single pulse, HBM-bound form) ----------
+// A workgroup of 512 lanes streams one TR x TC tile of a dipole block D_L
+// and produces, in one pass, the row partials of D_L x_{L+1} and the
+// column partials of D_L^T x_L.  Lane t covers rows (t & 15) + 16 i and
+// column pairs (t >> 4) + 32 j.
+constexpr int kGvThreads = 512;
+constexpr int kGvRowIter = 16;                 // i
+constexpr int kGvColPairIter = 4;              // j
+constexpr int kGvTR = 16 * kGvRowIter;         // 256 rows
+constexpr int kGvTC = 2 * 32 * kGvColPairIter; // 256 cols
+constexpr int kGvTileElems = kGvTR * kGvTC;
+
+struct TileJob {
+  int L;        // pair index (D_L couples block L and L+1)
+  int rb;       // row tile
+  int cb;       // column tile
+  int pad;
+  long long doff;  // element offset of the tile in the tiled dipole store
+};
+
+// Per-propagator description of the blocks and the partial-sum buffers.
+struct SysDesc {
+  int nb;
+  int N;              // sum of block sizes
+  int P;              // pulses
+  int gauge;          // 0 velocity ('v'), 1 length ('l')
+  int off[kMaxBlocks + 1];
+  // pair L: nrow_part[L] partial vectors for D_L x_{L+1} (length s_L) and
+  // ncol_part[L] partial vectors for D_L^T x_L (length s_{L+1}).
+  int nrow_part[kMaxBlocks];
+  int ncol_part[kMaxBlocks];
+  long long row_part_off[kMaxBlocks];  // double2 offset (already times P)
+  long long col_part_off[kMaxBlocks];
+};
+
+enum FieldMode : int { kFieldTable = 0, kFieldPulse = 1, kFieldConst = 2 };
+enum StepMode : int { kModeFixed = 0, kModeAdaptive = 1, kModeSchedule = 2,
+                      kModeRhs = 3 };
+enum PulseKind : int { kSine2 = 0, kGauss = 1, kPwl = 2 };
+
+// Device-side pulse::params (src/tdse/include/pulse.hpp:20-26) + PWL.
+struct PulseDev {
+  int kind;
+  int n_env;
+  long long env_off;   // offset into the propagator's envelope store
+  double Eo, w, cepd, a, b;
+  double t_end, scale;
+};
+
+// Per-pulse step control, resident in device memory and updated on device.
+struct Ctl {
+  double t;        // time at the start of the current attempt
+  double h;        // proposed step (controller state)
+  double hc;       // step used by the current attempt
+  double t_end;
+  double atol, rtol;
+  double Fc;       // constant field (kFieldConst)
+  double nrm;      // 2-norm of the last candidate state
+  double err;      // last error ratio
+  double hacc;     // size of the last accepted step
+  int mode;
+  int fmode;
+  int done;
+  int last;        // current attempt is the clipped final step
+  int accept;      // outcome of the last attempt
+  int step;        // fixed/schedule: steps taken
+  int nsteps;      // fixed/schedule: total steps
+  int accepted;
+  int rejected;
+  int attempts;
+  int max_attempts;
+  int status;      // 0 ok, -1 attempt limit hit
+  int nlog;        // log entries written in the current chunk
+  int pad;
+};
+
+struct LogEnt {
+  double t;    // time after the accepted step
+  double h;    // accepted step size
+  double pop;  // |c_track|^2 after renormalisation
+};
+
+}  // namespace mpci

@@ -1,0 +1,502 @@
+// HIP kernels of the MI355X tdse engine (gfx950).
+//
+//   k_relayout_gemv  one-time re-layout of a row-major dipole block into the
+//                    tile-contiguous store streamed by k_gemv_tile.
+//   k_gemv_tile      fused single-pulse RHS coupling: one HBM pass over D_L
+//                    gives both D_L x_{L+1} and D_L^T x_L (the 4 cblas_dgemv
+//                    calls per block of MatVecV/MatVecL, src/tdse/tdse.cpp:
+//                    28-66 and 99-136, read D four times).
+//   k_stage          finishes stage s: sums the partials, adds -iE x, scales
+//                    by the stage field, stores k_s and forms the next stage
+//                    input (boost::odeint generic_rk stage algebra); on the
+//                    last stage it forms the order-8 solution, the embedded
+//                    error ratio and |y|^2 partials.
+//   k_finalize       per-pulse reduction + on-device accept/reject control.
+//   k_apply          accepted: x = y / ||y|| (tdse.cpp:194-198) + log entry.
+#include "kernels.hpp"
+
+namespace mpci {
+
+// Fehlberg 7(8) tableau (NASA TR R-287, 1968), the coefficients of
+// boost::odeint runge_kutta_fehlberg78.
+__constant__ double cA[kStages][kStages] = {
+    {0},
+    {2.0 / 27.0},
+    {1.0 / 36.0, 1.0 / 12.0},
+    {1.0 / 24.0, 0.0, 1.0 / 8.0},
+    {5.0 / 12.0, 0.0, -25.0 / 16.0, 25.0 / 16.0},
+    {1.0 / 20.0, 0.0, 0.0, 1.0 / 4.0, 1.0 / 5.0},
+    {-25.0 / 108.0, 0.0, 0.0, 125.0 / 108.0, -65.0 / 27.0, 125.0 / 54.0},
+    {31.0 / 300.0, 0.0, 0.0, 0.0, 61.0 / 225.0, -2.0 / 9.0, 13.0 / 900.0},
+    {2.0, 0.0, 0.0, -53.0 / 6.0, 704.0 / 45.0, -107.0 / 9.0, 67.0 / 90.0, 3.0},
+    {-91.0 / 108.0, 0.0, 0.0, 23.0 / 108.0, -976.0 / 135.0, 311.0 / 54.0,
+     -19.0 / 60.0, 17.0 / 6.0, -1.0 / 12.0},
+    {2383.0 / 4100.0, 0.0, 0.0, -341.0 / 164.0, 4496.0 / 1025.0,
+     -301.0 / 82.0, 2133.0 / 4100.0, 45.0 / 82.0, 45.0 / 164.0, 18.0 / 41.0},
+    {3.0 / 205.0, 0.0, 0.0, 0.0, 0.0, -6.0 / 41.0, -3.0 / 205.0, -3.0 / 41.0,
+     3.0 / 41.0, 6.0 / 41.0, 0.0},
+    {-1777.0 / 4100.0, 0.0, 0.0, -341.0 / 164.0, 4496.0 / 1025.0,
+     -289.0 / 82.0, 2193.0 / 4100.0, 51.0 / 82.0, 33.0 / 164.0, 12.0 / 41.0,
+     0.0, 1.0},
+};
+__constant__ double cB[kStages] = {0, 0, 0, 0, 0, 34.0 / 105.0, 9.0 / 35.0,
+                                   9.0 / 35.0, 9.0 / 280.0, 9.0 / 280.0, 0,
+                                   41.0 / 840.0, 41.0 / 840.0};
+// b - bhat: the embedded error weights.
+__constant__ double cDB[kStages] = {-41.0 / 840.0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+                                    -41.0 / 840.0, 41.0 / 840.0,
+                                    41.0 / 840.0};
+__constant__ double cC[kStages] = {0, 2.0 / 27.0, 1.0 / 9.0, 1.0 / 6.0,
+                                   5.0 / 12.0, 0.5, 5.0 / 6.0, 1.0 / 6.0,
+                                   2.0 / 3.0, 1.0 / 3.0, 1.0, 0, 1.0};
+
+// ---------------------------------------------------------------------
+// Laser field, src/tdse/pulse.cpp:34-39 (gauss), 52-56 (sineAA), 93-97
+// (sineEE), plus the piecewise-linear envelope extension.
+// ---------------------------------------------------------------------
+__device__ double field_at(const PulseDev &p, const double *env, double t) {
+  if (p.kind == kSine2) {
+    double sinWt = sin(p.a * t);
+    return (double)(t >= 0.0 && t <= M_PI / p.a) * p.Eo * sinWt * sinWt *
+           sin(p.w * t + p.cepd);
+  }
+  if (p.kind == kGauss) {
+    double t_diff = t - 0.5 * p.b;
+    return (double)(t >= 0.0 && t <= p.b) * p.Eo *
+           exp(-t_diff * t_diff / (p.a * p.a)) * sin(p.w * t + p.cepd);
+  }
+  if (!(t >= 0.0 && t <= p.t_end) || p.n_env < 2) return 0.0;
+  const double *e = env + p.env_off;
+  double dx = p.t_end / (p.n_env - 1);
+  double u = t / dx;
+  int k = (int)floor(u);
+  if (k > p.n_env - 2) k = p.n_env - 2;
+  if (k < 0) k = 0;
+  double f = u - k;
+  double en = (1.0 - f) * e[k] + f * e[k + 1];
+  return p.scale * en * sin(p.w * t + p.cepd);
+}
+
+// ---------------------------------------------------------------------
+// One-time dipole re-layout into GEMV tiles.
+// Tile (rb, cb) of pair L holds element (i, j, t, e) at
+// ((i * 4 + j) * 512 + t) * 2 + e, which is D[rb*256 + (t&15) + 16 i]
+// [cb*256 + 2((t>>4) + 32 j) + e] or 0 outside the block.
+// ---------------------------------------------------------------------
+__global__ void k_relayout_gemv(const double *__restrict__ D, int sL, int sL1,
+                                int ncb, long long nelem,
+                                double *__restrict__ out) {
+  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  long long tile = e / kGvTileElems;
+  int within = (int)(e - tile * kGvTileElems);
+  int rb = (int)(tile / ncb), cb = (int)(tile % ncb);
+  int el = within & 1;
+  int q = within >> 1;
+  int t = q % kGvThreads;
+  int ij = q / kGvThreads;
+  int j = ij % kGvColPairIter, i = ij / kGvColPairIter;
+  int row = rb * kGvTR + (t & 15) + 16 * i;
+  int col = cb * kGvTC + 2 * ((t >> 4) + 32 * j) + el;
+  out[e] = (row < sL && col < sL1) ? D[(long long)row * sL1 + col] : 0.0;
+}
+
+// ---------------------------------------------------------------------
+// Fused single-pass coupling over one 256 x 256 tile of D_L.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kGvThreads) void k_gemv_tile(
+    const TileJob *__restrict__ jobs, const double *__restrict__ Dg,
+    const double2 *__restrict__ X, double2 *__restrict__ part, SysDesc d,
+    const Ctl *__restrict__ ctl) {
+  if (ctl[0].done) return;
+  const TileJob jb = jobs[blockIdx.x];
+  const int L = jb.L;
+  const int sL = d.off[L + 1] - d.off[L];
+  const int sL1 = d.off[L + 2] - d.off[L + 1];
+  const double2 *x0 = X + d.off[L];
+  const double2 *x1 = X + d.off[L + 1];
+  const int t = threadIdx.x;
+  const int tr = t & 15, g = t >> 4, lane = t & 63, w = t >> 6;
+  const int row0 = jb.rb * kGvTR, col0 = jb.cb * kGvTC;
+  const int nrows = min(kGvTR, sL - row0);
+
+  __shared__ double2 xs0[kGvTR];
+  __shared__ double2 rowbuf[kGvThreads / 64][kGvTR];
+
+  for (int r = t; r < kGvTR; r += kGvThreads)
+    xs0[r] = (r < nrows) ? x0[row0 + r] : make_double2(0.0, 0.0);
+
+  double2 xc[kGvColPairIter][2];
+  double2 acc[kGvColPairIter][2];
+  bool cv[kGvColPairIter];
+#pragma unroll
+  for (int j = 0; j < kGvColPairIter; ++j) {
+    const int c = col0 + 2 * (g + 32 * j);
+    cv[j] = c < sL1;
+    xc[j][0] = (c < sL1) ? x1[c] : make_double2(0.0, 0.0);
+    xc[j][1] = (c + 1 < sL1) ? x1[c + 1] : make_double2(0.0, 0.0);
+    acc[j][0] = make_double2(0.0, 0.0);
+    acc[j][1] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+
+  // Software-pipelined stream over the tile: the loads of row group i + 1
+  // are in flight while row group i is consumed.  Every load is issued
+  // unconditionally: a lane whose row or column pair lies outside the block
+  // re-reads its own j = 0 element of the group, which lies inside the
+  // zero-padded tile store and is fetched by the j = 0 load anyway.  Such a
+  // value only meets exact zeros (xs0 and xc are zero outside the block) or
+  // feeds partials that are never stored, so it needs no masking.
+  const double2 *Dt = reinterpret_cast<const double2 *>(Dg + jb.doff);
+  const int nI = (nrows + 15) >> 4;  // uniform over the workgroup
+  auto load_group = [&](int i, double2 (&v)[kGvColPairIter]) {
+    const bool rv = (tr + 16 * i) < nrows;
+    const double2 *base = Dt + i * kGvColPairIter * kGvThreads + t;
+#pragma unroll
+    for (int j = 0; j < kGvColPairIter; ++j) {
+      int o = (rv && cv[j]) ? j * kGvThreads : 0;
+      // opaque to the optimiser: keeps four independent vector loads
+      // instead of a branch that reuses the j = 0 load
+      asm volatile("" : "+v"(o));
+      v[j] = base[o];
+    }
+  };
+  double2 cur[kGvColPairIter], nxt[kGvColPairIter];
+  load_group(0, cur);
+  for (int i = 0; i < nI; ++i) {
+    // unconditional prefetch (the last one re-reads group nI-1 from cache)
+    // keeps the wait counters exact across iterations
+    load_group(min(i + 1, nI - 1), nxt);
+    const int r = tr + 16 * i;
+    const double2 xr = xs0[r];
+    double pre = 0.0, pim = 0.0;
+#pragma unroll
+    for (int j = 0; j < kGvColPairIter; ++j) {
+      const double2 dv = cur[j];
+      pre = fma(dv.x, xc[j][0].x, pre);
+      pim = fma(dv.x, xc[j][0].y, pim);
+      pre = fma(dv.y, xc[j][1].x, pre);
+      pim = fma(dv.y, xc[j][1].y, pim);
+      acc[j][0].x = fma(dv.x, xr.x, acc[j][0].x);
+      acc[j][0].y = fma(dv.x, xr.y, acc[j][0].y);
+      acc[j][1].x = fma(dv.y, xr.x, acc[j][1].x);
+      acc[j][1].y = fma(dv.y, xr.y, acc[j][1].y);
+    }
+    // sum the row partial over the 4 column groups held by this wave
+    pre += __shfl_xor(pre, 16);
+    pim += __shfl_xor(pim, 16);
+    pre += __shfl_xor(pre, 32);
+    pim += __shfl_xor(pim, 32);
+    if (lane < 16) rowbuf[w][r] = make_double2(pre, pim);
+#pragma unroll
+    for (int j = 0; j < kGvColPairIter; ++j) cur[j] = nxt[j];
+  }
+
+  // column partials: sum over the 16 row lanes (lane bits 0..3)
+#pragma unroll
+  for (int j = 0; j < kGvColPairIter; ++j) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      double ax = acc[j][e].x, ay = acc[j][e].y;
+#pragma unroll
+      for (int m = 1; m < 16; m <<= 1) {
+        ax += __shfl_xor(ax, m);
+        ay += __shfl_xor(ay, m);
+      }
+      acc[j][e].x = ax;
+      acc[j][e].y = ay;
+    }
+  }
+  if (tr == 0) {
+    double2 *pc = part + d.col_part_off[L] + (long long)jb.rb * sL1;
+#pragma unroll
+    for (int j = 0; j < kGvColPairIter; ++j) {
+      const int c = col0 + 2 * (g + 32 * j);
+      if (c < sL1) pc[c] = acc[j][0];
+      if (c + 1 < sL1) pc[c + 1] = acc[j][1];
+    }
+  }
+  __syncthreads();
+  // row partials: sum over the 8 waves in a fixed order
+  {
+    const int r = t >> 1, comp = t & 1;
+    if (r < nrows) {
+      double v = 0.0;
+#pragma unroll
+      for (int ww = 0; ww < kGvThreads / 64; ++ww)
+        v += comp ? rowbuf[ww][r].y : rowbuf[ww][r].x;
+      double *pr = reinterpret_cast<double *>(part + d.row_part_off[L] +
+                                              (long long)jb.cb * sL + row0 + r);
+      pr[comp] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// Stage kernel.  Thread (row, p) with rows_per_block = 256 / P.
+// ---------------------------------------------------------------------
+__device__ inline int block_of(const SysDesc &d, int row) {
+  int L = 0;
+  while (L + 1 < d.nb && row >= d.off[L + 1]) ++L;
+  return L;
+}
+
+__device__ inline double2 sum_partials(const double2 *base, int n,
+                                       long long stride, long long at) {
+  double2 acc = make_double2(0.0, 0.0);
+  for (int q0 = 0; q0 < n; q0 += 8) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      v[u] = (q0 + u < n) ? base[(long long)(q0 + u) * stride + at]
+                          : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      acc.x += v[u].x;
+      acc.y += v[u].y;
+    }
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
+  const int P = d.P;
+  const int rpb = 256 / P;
+  const int t = threadIdx.x;
+  const int lr = t / P, p = t - lr * P;
+  const int row = blockIdx.x * rpb + lr;
+  const bool active_lane = (lr < rpb) && (row < d.N);
+  const int s = a.s;
+  double errv = 0.0, nrm2 = 0.0;
+  if (active_lane) {
+    const Ctl &c = a.ctl[p];
+    if (!c.done) {
+      const int L = block_of(d, row);
+      const int i = row - d.off[L];
+      const int sL = d.off[L + 1] - d.off[L];
+      const long long idx = (long long)row * P + p;
+      double2 cs = make_double2(0.0, 0.0), rs = make_double2(0.0, 0.0);
+      // Partials are read 8 at a time so their loads are in flight together.
+      if (L > 0)
+        cs = sum_partials(a.part + d.col_part_off[L - 1], d.ncol_part[L - 1],
+                          (long long)sL * P, (long long)i * P + p);
+      if (L < d.nb - 1)
+        rs = sum_partials(a.part + d.row_part_off[L], d.nrow_part[L],
+                          (long long)sL * P, (long long)i * P + p);
+      const double2 xs = (s == 0) ? a.X[idx] : a.Xs[idx];
+      const double e = a.E[row];
+      double F;
+      if (c.fmode == kFieldTable)
+        F = a.Ftab[(long long)p * c.nsteps + c.step];
+      else if (c.fmode == kFieldPulse)
+        F = field_at(a.pulses[p], a.env, c.t + cC[s] * c.hc);
+      else
+        F = c.Fc;
+      double2 k;
+      if (d.gauge == 0) {  // velocity: -iE x + F (D^T x_{L-1} - D x_{L+1})
+        k.x = e * xs.y + F * (cs.x - rs.x);
+        k.y = -e * xs.x + F * (cs.y - rs.y);
+      } else {             // length: -i (E x + F (D^T x_{L-1} + D x_{L+1}))
+        const double cx = cs.x + rs.x, cy = cs.y + rs.y;
+        k.x = e * xs.y + F * cy;
+        k.y = -e * xs.x - F * cx;
+      }
+      const long long NP = a.NP;
+      a.K[s * NP + idx] = k;
+      if (c.mode != kModeRhs) {
+        const double h = c.hc;
+        const double2 x = a.X[idx];
+        if (s < kStages - 1) {
+          double vx = 1.0 * x.x, vy = 1.0 * x.y;
+          double2 kj[kStages - 1];
+#pragma unroll
+          for (int j = 0; j < kStages - 1; ++j)
+            kj[j] = (j < s) ? a.K[j * NP + idx] : make_double2(0.0, 0.0);
+#pragma unroll
+          for (int j = 0; j < kStages - 1; ++j) {
+            if (j < s) {
+              const double co = h * cA[s + 1][j];
+              vx += co * kj[j].x;
+              vy += co * kj[j].y;
+            }
+          }
+          const double co = h * cA[s + 1][s];
+          vx += co * k.x;
+          vy += co * k.y;
+          a.Xs[idx] = make_double2(vx, vy);
+        } else {
+          double yx = 1.0 * x.x, yy = 1.0 * x.y, ex = 0.0, ey = 0.0;
+          double2 kj[kStages];
+#pragma unroll
+          for (int j = 0; j < kStages - 1; ++j) kj[j] = a.K[j * NP + idx];
+          kj[kStages - 1] = k;
+          const double2 k1 = kj[0];
+#pragma unroll
+          for (int j = 0; j < kStages; ++j) {
+            const double cb = h * cB[j], cd = h * cDB[j];
+            yx += cb * kj[j].x;
+            yy += cb * kj[j].y;
+            ex += cd * kj[j].x;
+            ey += cd * kj[j].y;
+          }
+          a.Y[idx] = make_double2(yx, yy);
+          const double rx =
+              fabs(ex) / (c.atol + c.rtol * (fabs(x.x) + h * fabs(k1.x)));
+          const double ry =
+              fabs(ey) / (c.atol + c.rtol * (fabs(x.y) + h * fabs(k1.y)));
+          errv = rx > ry ? rx : ry;
+          nrm2 = yx * yx + yy * yy;
+        }
+      }
+    }
+  }
+  if (s == kStages - 1 && a.red != nullptr) {
+    __shared__ double se[256], sn[256];
+    se[t] = errv;
+    sn[t] = nrm2;
+    __syncthreads();
+    if (t < P) {
+      double em = 0.0, ns = 0.0;
+      for (int r = 0; r < rpb; ++r) {
+        const double v = se[r * P + t];
+        em = v > em ? v : em;
+        ns += sn[r * P + t];
+      }
+      a.red[((long long)blockIdx.x * P + t) * 2 + 0] = em;
+      a.red[((long long)blockIdx.x * P + t) * 2 + 1] = ns;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// Per-pulse reduction and step control; one workgroup per pulse.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_finalize(Ctl *__restrict__ ctl,
+                                                  const double *__restrict__ red,
+                                                  int nblk, int P,
+                                                  const double *__restrict__ hs) {
+  const int p = blockIdx.x, t = threadIdx.x;
+  if (ctl[p].done) {
+    if (t == 0) ctl[p].accept = 0;
+    return;
+  }
+  __shared__ double se[256], sn[256];
+  double em = 0.0, ns = 0.0;
+  for (int b = t; b < nblk; b += 256) {
+    const double e = red[((long long)b * P + p) * 2];
+    em = e > em ? e : em;
+    ns += red[((long long)b * P + p) * 2 + 1];
+  }
+  se[t] = em;
+  sn[t] = ns;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      se[t] = se[t] > se[t + w] ? se[t] : se[t + w];
+      sn[t] += sn[t + w];
+    }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  Ctl c = ctl[p];
+  const double E = se[0];
+  c.nrm = sqrt(sn[0]);
+  c.err = E;
+  c.attempts++;
+  if (c.mode == kModeAdaptive) {
+    const double hh = c.hc;
+    if (E > 1.0) {
+      // reject: h *= max(0.9 E^(-1/(q-1)), 1/5), error order q = 7
+      const double f = 0.9 * pow(E, -1.0 / 6.0);
+      c.h = hh * (f > 0.2 ? f : 0.2);
+      c.rejected++;
+      c.accept = 0;
+    } else {
+      c.accept = 1;
+      c.accepted++;
+      c.hacc = hh;
+      c.nlog++;
+      if (c.last) {
+        c.t = c.t_end;
+        c.done = 1;
+      } else {
+        c.t += hh;
+        if (E < 0.5) {
+          // increase: h *= 0.9 max(E, 5^-8)^(-1/8), stepper order 8
+          const double Em = E > 1.0 / 390625.0 ? E : 1.0 / 390625.0;
+          c.h = hh * (0.9 * pow(Em, -1.0 / 8.0));
+        } else {
+          c.h = hh;
+        }
+      }
+    }
+    if (!c.done && c.attempts >= c.max_attempts) {
+      c.done = 1;
+      c.status = -1;
+    }
+    if (!c.done) {
+      c.last = (c.t + c.h >= c.t_end);
+      c.hc = c.last ? (c.t_end - c.t) : c.h;
+    }
+  } else {
+    c.accept = 1;
+    c.accepted++;
+    c.hacc = c.hc;
+    c.nlog++;
+    c.t += c.hc;
+    c.step++;
+    if (c.step >= c.nsteps)
+      c.done = 1;
+    else if (c.mode == kModeSchedule)
+      c.hc = hs[(long long)p * c.nsteps + c.step];
+  }
+  ctl[p] = c;
+}
+
+__global__ __launch_bounds__(256) void k_apply(double2 *__restrict__ X,
+                                               const double2 *__restrict__ Y,
+                                               const Ctl *__restrict__ ctl,
+                                               LogEnt *__restrict__ log,
+                                               int logcap, int track, int N,
+                                               int P) {
+  const int rpb = 256 / P;
+  const int t = threadIdx.x;
+  const int lr = t / P, p = t - lr * P;
+  const int row = blockIdx.x * rpb + lr;
+  if (lr >= rpb || row >= N) return;
+  const Ctl &c = ctl[p];
+  if (!c.accept) return;
+  const long long idx = (long long)row * P + p;
+  const double2 y = Y[idx];
+  const double2 xn = make_double2(y.x / c.nrm, y.y / c.nrm);
+  X[idx] = xn;
+  if (row == track && log != nullptr && c.nlog >= 1 && c.nlog <= logcap) {
+    LogEnt le;
+    le.t = c.t;
+    le.h = c.hacc;
+    le.pop = xn.x * xn.x + xn.y * xn.y;
+    log[(long long)p * logcap + c.nlog - 1] = le;
+  }
+}
+
+// Weighted populations: out[o * P + p] = sum_i W[o][i] |x_{i,p}|^2.
+__global__ __launch_bounds__(256) void k_weighted_pop(
+    const double2 *__restrict__ X, const double *__restrict__ W, int N, int P,
+    double *__restrict__ out) {
+  const int o = blockIdx.y, p = blockIdx.x, t = threadIdx.x;
+  double acc = 0.0;
+  for (int i = t; i < N; i += 256) {
+    const double2 v = X[(long long)i * P + p];
+    acc += W[(long long)o * N + i] * (v.x * v.x + v.y * v.y);
+  }
+  __shared__ double s[256];
+  s[t] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) s[t] += s[t + w];
+    __syncthreads();
+  }
+  if (t == 0) out[o * P + p] = s[0];
+}
+
+}  // namespace mpci

@@ -1,0 +1,9 @@
+"""MI355X (gfx950) implementation of MP-CITDSE's `tdse` propagation path.
+
+All compute runs in lib/libmpci_tdse.so (hand-written HIP kernels behind the
+C ABI of include/mpci_tdse.h); this package is the Python-side mirror of
+that interface.
+"""
+from ._lib import MpciError, EXPORTED  # noqa: F401
+from .engine import (Propagator, Pulse, System, device_count,  # noqa: F401
+                     period, to_au)

@@ -139,7 +139,8 @@ class System:
         return x, t, pop[:steps], fld[:steps]
 
     def prop_adaptive(self, pulse, t0, t1, h0, atol, rtol, x,
-                      max_attempts=10_000_000, max_log=1_000_000):
+                      max_attempts=10_000_000, max_log=1_000_000,
+                      allow_limit=False):
         x = np.array(x, dtype=np.float64, copy=True)
         st = OrStats()
         hlog = np.empty(max_log)
@@ -148,7 +149,7 @@ class System:
                                     C.c_double(h0), C.c_double(atol),
                                     C.c_double(rtol), max_attempts, _ptr(x),
                                     C.byref(st), _ptr(hlog), max_log)
-        if rc != 0:
+        if rc != 0 and not allow_limit:
             raise RuntimeError("oracle adaptive: attempt limit reached")
         return x, st, hlog[:min(st.accepted, max_log)].copy()
 

@@ -1,0 +1,68 @@
+"""CPU-side checks of the C ABI: the built library loads without a GPU and
+exports every function include/mpci_tdse.h declares; the host-side pulse
+helpers agree with the oracle's restatement of src/tdse/pulse.cpp; the
+product refuses to run without a GPU instead of falling back to the CPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "mpci_tdse.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(mpci_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_header():
+    from mpci import _lib
+    lib = ctypes.CDLL(_lib.TDSE_LIB)
+    names = header_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(_lib.EXPORTED) == names
+
+
+def test_pulses_match_oracle(oracle):
+    import mpci
+    for gauge in "vl":
+        a = mpci.Pulse.sine(0.0009, 0.7, 10, gauge=gauge, cepd=0.1)
+        b = oracle.Pulse.sine(0.0009, 0.7, 10, gauge=gauge, cepd=0.1)
+        g1 = mpci.Pulse.gauss(0.0009, 0.7, 89.76, 10, gauge=gauge)
+        g2 = oracle.Pulse.gauss(0.0009, 0.7, 89.76, 10, gauge=gauge)
+        env = np.linspace(0, 0.04, 9)
+        p1 = mpci.Pulse.pwl(env, 0.7, 50.0, gauge=gauge)
+        p2 = oracle.Pulse.pwl(env, 0.7, 50.0, gauge=gauge)
+        for t in np.linspace(-1, 95, 301):
+            assert a(t) == b(t)
+            assert g1(t) == g2(t)
+            assert p1(t) == p2(t)
+    assert mpci.to_au(1e14, 15.0) == oracle.to_au(1e14, 15.0)
+
+
+def test_no_gpu_fails_loudly():
+    import mpci
+    if mpci.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(mpci.MpciError) as ei:
+        mpci.System([np.zeros(2), np.zeros(3)], [np.zeros((2, 3))])
+    assert ei.value.code == -2 and "GPU" in str(ei.value)
+
+
+def test_synthetic_generator_is_seeded():
+    from mpci import synth
+    E1, D1 = synth.system_arrays([50, 40, 30], seed=99)
+    E2, D2 = synth.system_arrays([50, 40, 30], seed=99)
+    E3, _ = synth.system_arrays([50, 40, 30], seed=98)
+    assert all(np.array_equal(a, b) for a, b in zip(E1, E2))
+    assert all(np.array_equal(a, b) for a, b in zip(D1, D2))
+    assert not np.array_equal(E1[0], E3[0])
+    for e in E1:
+        assert np.all(np.diff(e) >= 0) and e.min() >= -2.9 and e.max() < 5.0
+    d = synth.system_arrays([400, 400], seed=1)[1][0]
+    assert abs(d.std() * np.sqrt(400) - 1.0) < 0.01 and abs(d.mean()) < 2e-3
