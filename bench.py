@@ -63,7 +63,7 @@ def run_gpu(args, mpci, synth, sysg):
     st, _ = prop.prop_adaptive(0.0, tau, 0.05, TOL, TOL,
                                max_attempts=args.steps, allow_limit=True)
     wall = time.perf_counter() - t0
-    kms, klaunch, kbytes = prop.kernel_time()
+    kms, klaunch, kbytes, _ = prop.kernel_time()
     prop.set_instrument(False)
     s = st[0]
     return {"wall": wall, "accepted": s.accepted, "rejected": s.rejected,
@@ -88,6 +88,59 @@ def run_reference_mode(mpci, synth, sysg, steps=200, dt=0.1):
     prop.prop_fixed(0.0, dt, F)
     wall = time.perf_counter() - t0
     return {"steps_per_s": steps / wall, "steps": steps, "dt": dt}
+
+
+def run_batch(mpci, synth, sysg, P=64, attempts=200):
+    """Batched form: P pulses (w ~ U[0.5, 1], E0 log-uniform in [0.005,
+    0.05], 4-20 cycles) on the He system, per-pulse adaptive control; a
+    bounded number of lockstep attempts from t = 0."""
+    import numpy as np
+    w, E0, cyc = synth.pulse_batch(P)
+    prop = mpci.Propagator(sysg, P=P)
+    x0 = synth.ground_state(sysg.N)
+    t1 = np.array([mpci.period(w[p], int(cyc[p])) for p in range(P)])
+    for p in range(P):
+        prop.set_pulse(mpci.Pulse.sine(E0[p] ** 2, w[p], int(cyc[p])), p)
+    for p in range(P):
+        prop.set_state(x0, p)
+    prop.prop_adaptive(0.0, t1, 0.05, TOL, TOL, max_attempts=10,
+                       allow_limit=True)
+    for p in range(P):
+        prop.set_state(x0, p)
+    prop.set_instrument(True)
+    t0 = time.perf_counter()
+    st, _ = prop.prop_adaptive(0.0, t1, 0.05, TOL, TOL, max_attempts=attempts,
+                               allow_limit=True)
+    wall = time.perf_counter() - t0
+    kms, kl, _, kflops = prop.kernel_time()
+    prop.set_instrument(False)
+    acc = sum(x.accepted for x in st)
+    avg_ms = kms / max(kl, 1)
+    return {"pulse_steps_per_s": acc / wall, "P": P, "attempts": attempts,
+            "accepted_total": acc, "wall_s": wall,
+            "gemm": prop.gemm_config(),
+            "mfma_tflops": kflops / (avg_ms * 1e-3) / 1e12 if kl else None,
+            "avg_launch_us": avg_ms * 1e3}
+
+
+def run_adjoint(mpci, synth, sysg, B=16, K=256, cycles=20, w=0.7):
+    """Discrete-adjoint gradients of the final ground-state population with
+    respect to K piecewise-linear envelope samples (uniform in [0, 0.04]
+    a.u.) for a batch of B envelopes, checkpoints every 32 steps."""
+    import numpy as np
+    T = mpci.period(w, cycles)
+    prop = mpci.Propagator(sysg, P=B)
+    x0 = synth.ground_state(sysg.N)
+    W = np.zeros(sysg.N)
+    W[0] = 1.0
+    for p in range(B):
+        prop.set_state(x0, p)
+        prop.set_pulse(mpci.Pulse.pwl(synth.uniform(77 + p, 0, K, 0.0, 0.04),
+                                      w, T), p)
+    J, grad, nst, _, st = prop.adjoint(0.0, T, 0.05, W, ckpt_every=32)
+    return {"B": B, "samples": K, "cycles": cycles, "steps_max": st.steps_max,
+            "fwd_ms": st.fwd_ms, "bwd_ms": st.bwd_ms,
+            "adjoint_over_forward": st.bwd_ms / st.fwd_ms}
 
 
 def cpu_baseline(E, D, budget_s=12.0):
@@ -143,6 +196,8 @@ def main():
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build propagates on a "
@@ -192,6 +247,20 @@ def main():
         "reference_mode": {"steps_per_s": round(ref["steps_per_s"], 3),
                            "dt": ref["dt"], "steps": ref["steps"]},
     }
+    if not args.no_batch:
+        log("bench: batch-64 ...")
+        b = run_batch(mpci, synth, sysg)
+        b["per_pulse_speedup_vs_single"] = b["pulse_steps_per_s"] / value
+        if b["mfma_tflops"] is not None:
+            b["mfma_frac"] = b["mfma_tflops"] / 78.6
+        out["batch64"] = {k: (round(v, 4) if isinstance(v, float) else v)
+                          for k, v in b.items()}
+    if not args.no_adjoint:
+        log("bench: adjoint (16 envelopes x 256 samples) ...")
+        a = run_adjoint(mpci, synth, sysg)
+        out["adjoint"] = {k: (round(v, 4) if isinstance(v, float) else v)
+                          for k, v in a.items()}
+        out["adjoint_over_forward"] = round(a["adjoint_over_forward"], 4)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(E, D)

@@ -59,6 +59,13 @@ typedef struct mpci_stats {
   double h;       /* next proposed step */
 } mpci_stats;
 
+typedef struct mpci_adjoint_stats {
+  int steps_max;   /* most accepted steps of any pulse */
+  int segments;    /* checkpoint segments swept backward */
+  double fwd_ms;   /* forward pass (adaptive, with checkpoints) */
+  double bwd_ms;   /* segment recomputation + backward sweep */
+} mpci_adjoint_stats;
+
 typedef struct mpci_system mpci_system;
 typedef struct mpci_prop mpci_prop;
 
@@ -94,6 +101,9 @@ int mpci_system_destroy(mpci_system *sys);
 int mpci_system_info(const mpci_system *sys, int *N, long long *bytes);
 
 /* ---- propagator ------------------------------------------------------ */
+/* P pulses (1 <= P <= 256).  P = 1 streams D once per RHS through the fused
+ * GEMV kernel (HBM-bound); P >= 2 runs the coupling as FP64 MFMA GEMMs with
+ * the tile shape picked by an on-device autotuner at creation. */
 int mpci_prop_create(mpci_system *sys, int P, mpci_prop **out);
 int mpci_prop_destroy(mpci_prop *prop);
 /* ct: 2N doubles for pulse p (tdse_main.cpp:148-160 initial state). */
@@ -131,6 +141,24 @@ int mpci_prop_adaptive(mpci_prop *prop, const double *t0, const double *t1,
 int mpci_prop_schedule(mpci_prop *prop, const double *t0, int n,
                        const double *hs);
 
+/* Discrete adjoint (extension; the reference has none).  All P pulses must
+ * be MPCI_PULSE_PWL.  Runs the adaptive forward propagation of
+ * mpci_prop_adaptive from the current states, keeping the state after every
+ * ckpt_every accepted steps, then sweeps backward through the same discrete
+ * map (accepted steps only, schedule frozen, renormalisation included),
+ * recomputing each checkpoint segment.  Objective per pulse:
+ * J[p] = sum_k W[k] |c_{k,p}(t1[p])|^2 (e.g. the ground-state population or
+ * the yield mask of pes::genPES2e, src/pes/pes.cpp:199-213).  Output:
+ * grad[p * grad_stride + k] = dJ[p] / d env_p[k]; nsteps[p] and hlog
+ * (P x hlog_cap, may be NULL) return the accepted step sizes for
+ * frozen-schedule finite differences.  Leaves the propagator at the final
+ * states.  Fails with MPCI_ERR_LIMIT beyond max_steps accepted steps. */
+int mpci_prop_adjoint(mpci_prop *prop, const double *t0, const double *t1,
+                      const double *h0, double atol, double rtol,
+                      int max_steps, const double *W, int ckpt_every,
+                      double *J, double *grad, int grad_stride, int *nsteps,
+                      double *hlog, int hlog_cap, mpci_adjoint_stats *stats);
+
 /* On-device weighted populations: out[o * P + p] = sum_k W[o*N + k]
  * |c_{k,p}|^2 (ground population, yield and excitation masks of
  * pes::genPES2e, src/pes/pes.cpp:199-213; all-ones for the norm). */
@@ -145,7 +173,13 @@ int mpci_prop_set_graph(mpci_prop *prop, int enable);
  * mpci_prop_kernel_time.  Off by default. */
 int mpci_prop_set_instrument(mpci_prop *prop, int enable);
 int mpci_prop_kernel_time(mpci_prop *prop, double *total_ms,
-                          long long *launches, long long *bytes_per_launch);
+                          long long *launches, long long *bytes_per_launch,
+                          long long *flops_per_launch);
+/* Batched form (P >= 2): the MFMA tile (waves along M and N, each wave a
+ * 32 x 32 block) and K split the autotuner chose at mpci_prop_create, and
+ * its measured time per coupling evaluation in ms. */
+int mpci_prop_gemm_config(mpci_prop *prop, int *wm, int *wn, int *ks,
+                          double *ms);
 
 #ifdef __cplusplus
 }

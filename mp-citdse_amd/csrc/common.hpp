@@ -36,7 +36,22 @@ struct TileJob {
   int rb;       // row tile
   int cb;       // column tile
   int pad;
-  long long doff;  // element offset of the tile in the tiled dipole store
+  long long doff;  // job index (the tile's slot in the interleaved store)
+};
+
+// ---- batched GEMM form (P >= 2 pulses, FP64 MFMA) ----------------------
+// C = op(D_L) X over a K range, op = identity (dir 0: rows of block L, from
+// X_{L+1}) or transpose (dir 1: rows of block L+1, from X_L).  X is the
+// state viewed as a row-major matrix with 2P real columns.
+constexpr int kGmTK = 32;  // K chunk staged in LDS
+struct GemmJob {
+  int L;
+  int dir;
+  int m0;       // first output row of the panel
+  int n0;       // first real column
+  int k0, k1;   // K range [k0, k1)
+  int q;        // partial slot (K-split index)
+  int pad;
 };
 
 // Per-propagator description of the blocks and the partial-sum buffers.
@@ -92,7 +107,17 @@ struct Ctl {
   int max_attempts;
   int status;      // 0 ok, -1 attempt limit hit
   int nlog;        // log entries written in the current chunk
-  int pad;
+  int hstride;     // row stride of the per-pulse step / field tables
+};
+
+// Optional stores written while stepping (adjoint support).
+struct StepStore {
+  double2 *xs;      // stage inputs 1..12 of every step: [step][12][N x P]
+  double2 *xn;      // normalised states after every step: [step + 1][N x P]
+  double *nrm;      // ||y|| of every step: [step][P]
+  double2 *ckpt;    // adaptive mode: state after every ckpt_every accepted
+  int ckpt_every;   //   steps, slot accepted / ckpt_every, ckpt_cap slots
+  int ckpt_cap;
 };
 
 struct LogEnt {

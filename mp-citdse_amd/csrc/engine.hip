@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -80,6 +81,11 @@ struct mpci_system {
   std::vector<int> nrb, ncb;   // per pair: row tiles, column tiles
   long long S = 0;             // sum_L s_L s_{L+1} (dipole elements)
   int device = 0;
+  // row-major copy of every D_L for the batched (GEMM) form, built on the
+  // first propagator with P >= 2 from the host blocks kept until then
+  DevBuf<double> Dr;
+  DevBuf<long long> dr_off;
+  std::vector<std::vector<double>> hostD;
 };
 
 struct mpci_prop {
@@ -99,12 +105,18 @@ struct mpci_prop {
   DevBuf<LogEnt> log;
   int logcap = 4096;
   int track = 0;
+  StepStore store{};  // adjoint support: checkpoints and recompute stores
   std::vector<PulseDev> hpulses;
   std::vector<std::vector<double>> henv;
   bool use_graph = true;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int graph_track = -1;
+  // batched form: chosen MFMA tile (waves along M, N) and K splits
+  int gm_wm = 2, gm_wn = 2, gm_ks = 1;
+  double gemm_ms = 0.0;  // autotuned time of one coupling evaluation
+  std::vector<GemmJob> gjobs;
+  DevBuf<GemmJob> dgjobs;
   bool instrument = false;
   std::vector<hipEvent_t> ev;
   int ev_base = 0;
@@ -117,6 +129,25 @@ struct mpci_prop {
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
+
+// ---- copies on the propagator's stream ----------------------------------
+// Every host <-> device copy that touches buffers the propagator's kernels
+// use goes through its own stream and is waited for: a synchronous
+// hipMemcpy from pageable memory may return before its DMA has landed and
+// is not ordered with a non-blocking stream.
+static int h2d(mpci_prop *pr, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return MPCI_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, pr->stream));
+  HIPCHK(hipStreamSynchronize(pr->stream));
+  return MPCI_OK;
+}
+
+static int d2h(mpci_prop *pr, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return MPCI_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, pr->stream));
+  HIPCHK(hipStreamSynchronize(pr->stream));
+  return MPCI_OK;
+}
 
 // ---------------------------------------------------------------------
 // pulses (host): src/tdse/pulse.cpp
@@ -207,30 +238,13 @@ static int build_system(mpci_system *sys, const double *const *E,
   const int npair = sys->nb - 1;
   sys->nrb.assign(std::max(npair, 0), 0);
   sys->ncb.assign(std::max(npair, 0), 0);
-  long long total = 0;
-  std::vector<long long> base(std::max(npair, 0));
+  std::vector<int> job0(std::max(npair, 0));
+  sys->jobs.clear();
   for (int L = 0; L < npair; ++L) {
     sys->nrb[L] = (sys->s[L] + kGvTR - 1) / kGvTR;
     sys->ncb[L] = (sys->s[L + 1] + kGvTC - 1) / kGvTC;
-    base[L] = total;
-    total += (long long)sys->nrb[L] * sys->ncb[L] * kGvTileElems;
     sys->S += (long long)sys->s[L] * sys->s[L + 1];
-  }
-  TRY(sys->Dg.alloc(total));
-  sys->jobs.clear();
-  for (int L = 0; L < npair; ++L) {
-    const long long ne = (long long)sys->s[L] * sys->s[L + 1];
-    DevBuf<double> tmp;
-    TRY(tmp.alloc(ne));
-    HIPCHK(hipMemcpy(tmp.p, D[L], sizeof(double) * ne, hipMemcpyHostToDevice));
-    const long long nout = (long long)sys->nrb[L] * sys->ncb[L] * kGvTileElems;
-    const int tpb = 256;
-    const long long nb = (nout + tpb - 1) / tpb;
-    hipLaunchKernelGGL(k_relayout_gemv, dim3((unsigned)nb), dim3(tpb), 0, 0,
-                       tmp.p, sys->s[L], sys->s[L + 1], sys->ncb[L], nout,
-                       sys->Dg.p + base[L]);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipDeviceSynchronize());
+    job0[L] = (int)sys->jobs.size();
     for (int rb = 0; rb < sys->nrb[L]; ++rb)
       for (int cb = 0; cb < sys->ncb[L]; ++cb) {
         TileJob j;
@@ -238,15 +252,34 @@ static int build_system(mpci_system *sys, const double *const *E,
         j.rb = rb;
         j.cb = cb;
         j.pad = 0;
-        j.doff = base[L] + ((long long)rb * sys->ncb[L] + cb) * kGvTileElems;
+        j.doff = (long long)sys->jobs.size();
         sys->jobs.push_back(j);
       }
+  }
+  const int njobs = (int)sys->jobs.size();
+  TRY(sys->Dg.alloc((size_t)njobs * kGvTileElems));
+  for (int L = 0; L < npair; ++L) {
+    const long long ne = (long long)sys->s[L] * sys->s[L + 1];
+    DevBuf<double> tmp;
+    TRY(tmp.alloc(ne));
+    HIPCHK(hipMemcpy(tmp.p, D[L], sizeof(double) * ne, hipMemcpyHostToDevice));
+    const long long nout = (long long)sys->nrb[L] * sys->ncb[L] * kGvTileElems;
+    const int tpb = 256;
+    const long long nblocks = (nout + tpb - 1) / tpb;
+    hipLaunchKernelGGL(k_relayout_gemv, dim3((unsigned)nblocks), dim3(tpb), 0,
+                       0, tmp.p, sys->s[L], sys->s[L + 1], sys->ncb[L], nout,
+                       job0[L], njobs, sys->Dg.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
   }
   TRY(sys->djobs.alloc(sys->jobs.size()));
   if (!sys->jobs.empty())
     HIPCHK(hipMemcpy(sys->djobs.p, sys->jobs.data(),
                      sizeof(TileJob) * sys->jobs.size(),
                      hipMemcpyHostToDevice));
+  // pageable copies may still be in flight: finish them before any stream
+  // of a propagator reads the system
+  HIPCHK(hipDeviceSynchronize());
   return MPCI_OK;
 }
 
@@ -286,6 +319,10 @@ extern "C" int mpci_system_create(int nb, const int *s, char gauge,
     delete sys;
     return rc;
   }
+  // keep the host blocks for a later batched propagator
+  sys->hostD.resize(nb - 1);
+  for (int L = 0; L + 1 < nb; ++L)
+    sys->hostD[L].assign(D[L], D[L] + (size_t)s[L] * s[L + 1]);
   *out = sys;
   return MPCI_OK;
 }
@@ -308,7 +345,10 @@ extern "C" int mpci_system_info(const mpci_system *sys, int *N,
 // ---------------------------------------------------------------------
 // propagator
 // ---------------------------------------------------------------------
-static int init_desc(mpci_prop *pr) {
+// Partial-sum layout: the GEMV form writes ncb(L) row partials and nrb(L)
+// column partials per pair; the GEMM form writes `ks` K-split partials per
+// pair and direction.
+static int init_desc(mpci_prop *pr, int ks) {
   mpci_system *sys = pr->sys;
   SysDesc &d = pr->desc;
   std::memset(&d, 0, sizeof(d));
@@ -319,23 +359,142 @@ static int init_desc(mpci_prop *pr) {
   for (int L = 0; L <= sys->nb; ++L) d.off[L] = sys->off[L];
   long long o = 0;
   for (int L = 0; L + 1 < sys->nb; ++L) {
-    d.nrow_part[L] = sys->ncb[L];
-    d.ncol_part[L] = sys->nrb[L];
+    d.nrow_part[L] = pr->P == 1 ? sys->ncb[L] : ks;
+    d.ncol_part[L] = pr->P == 1 ? sys->nrb[L] : ks;
     d.row_part_off[L] = o;
-    o += (long long)sys->ncb[L] * sys->s[L] * pr->P;
+    o += (long long)d.nrow_part[L] * sys->s[L] * pr->P;
     d.col_part_off[L] = o;
-    o += (long long)sys->nrb[L] * sys->s[L + 1] * pr->P;
+    o += (long long)d.ncol_part[L] * sys->s[L + 1] * pr->P;
   }
   TRY(pr->part.alloc(std::max<long long>(o, 1)));
+  return MPCI_OK;
+}
+
+static int ensure_rowmajor(mpci_system *sys) {
+  if (sys->Dr.p || sys->nb < 2) return MPCI_OK;
+  std::vector<long long> off(sys->nb - 1);
+  long long tot = 0;
+  for (int L = 0; L + 1 < sys->nb; ++L) {
+    off[L] = tot;
+    tot += (long long)sys->s[L] * sys->s[L + 1];
+  }
+  TRY(sys->Dr.alloc(tot));
+  TRY(sys->dr_off.alloc(off.size()));
+  HIPCHK(hipMemcpy(sys->dr_off.p, off.data(), sizeof(long long) * off.size(),
+                   hipMemcpyHostToDevice));
+  for (int L = 0; L + 1 < sys->nb; ++L)
+    HIPCHK(hipMemcpy(sys->Dr.p + off[L], sys->hostD[L].data(),
+                     sizeof(double) * sys->hostD[L].size(),
+                     hipMemcpyHostToDevice));
+  HIPCHK(hipDeviceSynchronize());
+  return MPCI_OK;
+}
+
+// GEMM jobs: for every pair and direction, output panels of 32*wm rows x
+// 32*wn real columns, each over one of `ks` equal K ranges.
+static int build_gemm_jobs(mpci_prop *pr, int wm, int wn, int ks) {
+  mpci_system *sys = pr->sys;
+  const int TM = 32 * wm, TN = 32 * wn, P2 = 2 * pr->P;
+  pr->gjobs.clear();
+  for (int L = 0; L + 1 < sys->nb; ++L)
+    for (int dir = 0; dir < 2; ++dir) {
+      const int M = dir == 0 ? sys->s[L] : sys->s[L + 1];
+      const int K = dir == 0 ? sys->s[L + 1] : sys->s[L];
+      const int kchunk = ((K + ks - 1) / ks + kGmTK - 1) / kGmTK * kGmTK;
+      for (int n0 = 0; n0 < P2; n0 += TN)
+        for (int q = 0; q < ks; ++q)
+          for (int m0 = 0; m0 < M; m0 += TM) {
+            GemmJob j{};
+            j.L = L;
+            j.dir = dir;
+            j.m0 = m0;
+            j.n0 = n0;
+            j.k0 = std::min(K, q * kchunk);
+            j.k1 = std::min(K, (q + 1) * kchunk);
+            j.q = q;
+            pr->gjobs.push_back(j);
+          }
+    }
+  TRY(pr->dgjobs.alloc(pr->gjobs.size()));
+  if (!pr->gjobs.empty())
+    TRY(h2d(pr, pr->dgjobs.p, pr->gjobs.data(),
+            sizeof(GemmJob) * pr->gjobs.size()));
+  pr->gm_wm = wm;
+  pr->gm_wn = wn;
+  pr->gm_ks = ks;
+  return MPCI_OK;
+}
+
+static int launch_gemm(mpci_prop *pr, const double2 *xin) {
+  mpci_system *sys = pr->sys;
+  const dim3 grid((unsigned)pr->gjobs.size());
+  const double *X = reinterpret_cast<const double *>(xin);
+  double *part = reinterpret_cast<double *>(pr->part.p);
+#define MPCI_GEMM(WM, WN)                                                    \
+  hipLaunchKernelGGL((k_gemm_coupling<WM, WN>), grid, dim3(64 * WM * WN), 0, \
+                     pr->stream, pr->dgjobs.p, sys->Dr.p, sys->dr_off.p, X,  \
+                     part, pr->desc, pr->ctl.p)
+  if (pr->gm_wm == 2 && pr->gm_wn == 2)
+    MPCI_GEMM(2, 2);
+  else if (pr->gm_wm == 2 && pr->gm_wn == 4)
+    MPCI_GEMM(2, 4);
+  else if (pr->gm_wm == 4 && pr->gm_wn == 2)
+    MPCI_GEMM(4, 2);
+  else
+    MPCI_GEMM(4, 4);
+#undef MPCI_GEMM
+  HIPCHK(hipGetLastError());
+  return MPCI_OK;
+}
+
+// Autotuner of the batched form: times every (wm, wn, ks) candidate on this
+// system and pulse count (a few coupling evaluations each, HIP events on
+// the propagator's stream) and keeps the fastest.
+static int autotune_gemm(mpci_prop *pr) {
+  struct Cand {
+    int wm, wn, ks;
+  };
+  const Cand cands[] = {{2, 2, 1}, {2, 2, 2}, {2, 4, 1}, {2, 4, 2},
+                        {4, 2, 1}, {4, 2, 2}, {4, 4, 1}, {4, 4, 2},
+                        {4, 4, 4}, {2, 4, 4}};
+  std::vector<Ctl> c(pr->P);
+  std::memset(c.data(), 0, sizeof(Ctl) * pr->P);  // all pulses live
+  TRY(h2d(pr, pr->ctl.p, c.data(), sizeof(Ctl) * pr->P));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 1e30f;
+  Cand bc = cands[0];
+  for (const Cand &cd : cands) {
+    TRY(init_desc(pr, cd.ks));
+    TRY(build_gemm_jobs(pr, cd.wm, cd.wn, cd.ks));
+    TRY(launch_gemm(pr, pr->X.p));  // warm
+    HIPCHK(hipEventRecord(e0, pr->stream));
+    for (int r = 0; r < 3; ++r) TRY(launch_gemm(pr, pr->X.p));
+    HIPCHK(hipEventRecord(e1, pr->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      bc = cd;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (auto &x : c) x.done = 1;
+  TRY(h2d(pr, pr->ctl.p, c.data(), sizeof(Ctl) * pr->P));
+  TRY(init_desc(pr, bc.ks));
+  TRY(build_gemm_jobs(pr, bc.wm, bc.wn, bc.ks));
+  pr->gemm_ms = best / 3.0;
   return MPCI_OK;
 }
 
 extern "C" int mpci_prop_create(mpci_system *sys, int P, mpci_prop **out) {
   if (!out || !sys) return fail(MPCI_ERR_ARG, "NULL argument");
   *out = nullptr;
-  if (P != 1)
-    return fail(MPCI_ERR_NOTSUP, "this build propagates P = 1 pulse per "
-                                 "propagator");
+  if (P < 1 || P > 256)
+    return fail(MPCI_ERR_ARG, "P must be in [1, 256]");
   std::unique_ptr<mpci_prop> pr(new (std::nothrow) mpci_prop());
   if (!pr) return fail(MPCI_ERR_HIP, "host allocation failed");
   pr->sys = sys;
@@ -345,7 +504,7 @@ extern "C" int mpci_prop_create(mpci_system *sys, int P, mpci_prop **out) {
   pr->NP = (long long)sys->N * P;
   const int rpb = 256 / P;
   pr->nblk = (sys->N + rpb - 1) / rpb;
-  TRY(init_desc(pr.get()));
+  TRY(init_desc(pr.get(), 1));
   TRY(pr->X.alloc(pr->NP));
   TRY(pr->Xs.alloc(pr->NP));
   TRY(pr->Y.alloc(pr->NP));
@@ -354,15 +513,20 @@ extern "C" int mpci_prop_create(mpci_system *sys, int P, mpci_prop **out) {
   TRY(pr->ctl.alloc(P));
   TRY(pr->pulses.alloc(P));
   TRY(pr->log.alloc((size_t)pr->logcap * P));
-  HIPCHK(hipMemset(pr->X.p, 0, sizeof(double2) * pr->NP));
-  HIPCHK(hipMemset(pr->part.p, 0, sizeof(double2) * pr->part.n));
+  HIPCHK(hipMemsetAsync(pr->X.p, 0, sizeof(double2) * pr->NP, pr->stream));
+  HIPCHK(hipMemsetAsync(pr->part.p, 0, sizeof(double2) * pr->part.n,
+                        pr->stream));
+  HIPCHK(hipStreamSynchronize(pr->stream));
   std::vector<Ctl> c(P);
   std::memset(c.data(), 0, sizeof(Ctl) * P);
   for (auto &x : c) x.done = 1;
-  HIPCHK(hipMemcpy(pr->ctl.p, c.data(), sizeof(Ctl) * P,
-                   hipMemcpyHostToDevice));
+  TRY(h2d(pr.get(), pr->ctl.p, c.data(), sizeof(Ctl) * P));
   pr->hpulses.assign(P, PulseDev{});
   pr->henv.assign(P, {});
+  if (P > 1 && sys->nb > 1) {
+    TRY(ensure_rowmajor(sys));
+    TRY(autotune_gemm(pr.get()));
+  }
   *out = pr.release();
   return MPCI_OK;
 }
@@ -379,13 +543,13 @@ extern "C" int mpci_prop_set_state(mpci_prop *pr, int p, const double *ct) {
   if (!pr || !ct || p < 0 || p >= pr->P)
     return fail(MPCI_ERR_ARG, "mpci_prop_set_state: bad argument");
   if (pr->P == 1) {
-    HIPCHK(hipMemcpy(pr->X.p, ct, sizeof(double2) * pr->sys->N,
-                     hipMemcpyHostToDevice));
+    return h2d(pr, pr->X.p, ct, sizeof(double2) * pr->sys->N);
     return MPCI_OK;
   }
-  HIPCHK(hipMemcpy2D(pr->X.p + p, sizeof(double2) * pr->P, ct,
-                     sizeof(double2), sizeof(double2), pr->sys->N,
-                     hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy2DAsync(pr->X.p + p, sizeof(double2) * pr->P, ct,
+                          sizeof(double2), sizeof(double2), pr->sys->N,
+                          hipMemcpyHostToDevice, pr->stream));
+  HIPCHK(hipStreamSynchronize(pr->stream));
   return MPCI_OK;
 }
 
@@ -394,13 +558,13 @@ extern "C" int mpci_prop_get_state(mpci_prop *pr, int p, double *ct) {
     return fail(MPCI_ERR_ARG, "mpci_prop_get_state: bad argument");
   HIPCHK(hipStreamSynchronize(pr->stream));
   if (pr->P == 1) {
-    HIPCHK(hipMemcpy(ct, pr->X.p, sizeof(double2) * pr->sys->N,
-                     hipMemcpyDeviceToHost));
+    return d2h(pr, ct, pr->X.p, sizeof(double2) * pr->sys->N);
     return MPCI_OK;
   }
-  HIPCHK(hipMemcpy2D(ct, sizeof(double2), pr->X.p + p,
-                     sizeof(double2) * pr->P, sizeof(double2), pr->sys->N,
-                     hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy2DAsync(ct, sizeof(double2), pr->X.p + p,
+                          sizeof(double2) * pr->P, sizeof(double2), pr->sys->N,
+                          hipMemcpyDeviceToHost, pr->stream));
+  HIPCHK(hipStreamSynchronize(pr->stream));
   return MPCI_OK;
 }
 
@@ -415,11 +579,9 @@ static int upload_pulses(mpci_prop *pr) {
     std::vector<double> flat;
     flat.reserve(tot);
     for (auto &v : pr->henv) flat.insert(flat.end(), v.begin(), v.end());
-    HIPCHK(hipMemcpy(pr->env.p, flat.data(), sizeof(double) * tot,
-                     hipMemcpyHostToDevice));
+    TRY(h2d(pr, pr->env.p, flat.data(), sizeof(double) * tot));
   }
-  HIPCHK(hipMemcpy(pr->pulses.p, pr->hpulses.data(),
-                   sizeof(PulseDev) * pr->P, hipMemcpyHostToDevice));
+  TRY(h2d(pr, pr->pulses.p, pr->hpulses.data(), sizeof(PulseDev) * pr->P));
   return MPCI_OK;
 }
 
@@ -452,10 +614,14 @@ static int enqueue_tile(mpci_prop *pr, const double2 *xin, int s) {
   if (sys->jobs.empty()) return MPCI_OK;
   const bool ev = pr->instrument;
   if (ev) HIPCHK(hipEventRecord(pr->ev[pr->ev_base + 2 * s], pr->stream));
-  hipLaunchKernelGGL(k_gemv_tile, dim3((unsigned)sys->jobs.size()),
-                     dim3(kGvThreads), 0, pr->stream, sys->djobs.p, sys->Dg.p,
-                     xin, pr->part.p, pr->desc, pr->ctl.p);
-  HIPCHK(hipGetLastError());
+  if (pr->P == 1) {
+    hipLaunchKernelGGL(k_gemv_tile, dim3((unsigned)sys->jobs.size()),
+                       dim3(kGvThreads), 0, pr->stream, sys->djobs.p,
+                       sys->Dg.p, xin, pr->part.p, pr->desc, pr->ctl.p);
+    HIPCHK(hipGetLastError());
+  } else {
+    TRY(launch_gemm(pr, xin));
+  }
   if (ev) HIPCHK(hipEventRecord(pr->ev[pr->ev_base + 2 * s + 1], pr->stream));
   return MPCI_OK;
 }
@@ -473,6 +639,7 @@ static StageArgs stage_args(mpci_prop *pr, int s) {
   a.pulses = pr->pulses.p;
   a.env = pr->env.p;
   a.Ftab = pr->Ftab.p;
+  a.xs_store = pr->store.xs;
   a.NP = pr->NP;
   a.s = s;
   return a;
@@ -490,7 +657,7 @@ static int enqueue_attempt(mpci_prop *pr) {
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_apply, dim3(pr->nblk), dim3(256), 0, pr->stream,
                      pr->X.p, pr->Y.p, pr->ctl.p, pr->log.p, pr->logcap,
-                     pr->track, pr->sys->N, pr->P);
+                     pr->track, pr->sys->N, pr->P, pr->store);
   HIPCHK(hipGetLastError());
   return MPCI_OK;
 }
@@ -584,14 +751,12 @@ static int run_attempts(mpci_prop *pr, int n) {
 static int read_ctl(mpci_prop *pr, std::vector<Ctl> &c) {
   c.resize(pr->P);
   HIPCHK(hipStreamSynchronize(pr->stream));
-  HIPCHK(hipMemcpy(c.data(), pr->ctl.p, sizeof(Ctl) * pr->P,
-                   hipMemcpyDeviceToHost));
+  TRY(d2h(pr, c.data(), pr->ctl.p, sizeof(Ctl) * pr->P));
   return MPCI_OK;
 }
 
 static int write_ctl(mpci_prop *pr, const std::vector<Ctl> &c) {
-  HIPCHK(hipMemcpy(pr->ctl.p, c.data(), sizeof(Ctl) * pr->P,
-                   hipMemcpyHostToDevice));
+  TRY(h2d(pr, pr->ctl.p, c.data(), sizeof(Ctl) * pr->P));
   return MPCI_OK;
 }
 
@@ -604,8 +769,7 @@ static int drain_log(mpci_prop *pr, std::vector<Ctl> &c,
     return fail(MPCI_ERR_STATE, "internal: log overflow");
   if (maxn > 0) {
     std::vector<LogEnt> h((size_t)pr->logcap * pr->P);
-    HIPCHK(hipMemcpy(h.data(), pr->log.p, sizeof(LogEnt) * h.size(),
-                     hipMemcpyDeviceToHost));
+    TRY(d2h(pr, h.data(), pr->log.p, sizeof(LogEnt) * h.size()));
     for (int p = 0; p < pr->P; ++p)
       acc[p].insert(acc[p].end(), h.begin() + (size_t)p * pr->logcap,
                     h.begin() + (size_t)p * pr->logcap + c[p].nlog);
@@ -635,8 +799,7 @@ extern "C" int mpci_prop_rhs(mpci_prop *pr, const double *F, const double *x,
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(pr->stream));
   std::vector<double2> k(pr->NP);
-  HIPCHK(hipMemcpy(k.data(), pr->K.p, sizeof(double2) * pr->NP,
-                   hipMemcpyDeviceToHost));
+  TRY(d2h(pr, k.data(), pr->K.p, sizeof(double2) * pr->NP));
   const int N = pr->sys->N;
   for (int p = 0; p < pr->P; ++p)
     for (int i = 0; i < N; ++i) {
@@ -654,8 +817,7 @@ extern "C" int mpci_prop_fixed(mpci_prop *pr, double t0, double dt, int steps,
     return fail(MPCI_ERR_ARG, "track index out of range");
   if (steps == 0) return MPCI_OK;
   TRY(pr->Ftab.alloc((size_t)pr->P * steps));
-  HIPCHK(hipMemcpy(pr->Ftab.p, F, sizeof(double) * pr->P * steps,
-                   hipMemcpyHostToDevice));
+  TRY(h2d(pr, pr->Ftab.p, F, sizeof(double) * pr->P * steps));
   pr->track = track;
   drop_graph(pr);  // Ftab moved
   std::vector<Ctl> c(pr->P);
@@ -667,6 +829,7 @@ extern "C" int mpci_prop_fixed(mpci_prop *pr, double t0, double dt, int steps,
     x.h = dt;
     x.hc = dt;
     x.nsteps = steps;
+    x.hstride = steps;
     x.max_attempts = steps;
   }
   TRY(write_ctl(pr, c));
@@ -716,11 +879,10 @@ static int run_controlled(mpci_prop *pr, std::vector<Ctl> &c,
   return MPCI_OK;
 }
 
-extern "C" int mpci_prop_adaptive(mpci_prop *pr, const double *t0,
-                                  const double *t1, const double *h0,
-                                  double atol, double rtol, int max_attempts,
-                                  int track, mpci_stats *stats, double *log,
-                                  int log_cap) {
+static int run_adaptive(mpci_prop *pr, const double *t0, const double *t1,
+                        const double *h0, double atol, double rtol,
+                        int max_attempts, int track, std::vector<Ctl> &c,
+                        std::vector<std::vector<LogEnt>> &acc) {
   if (!pr || !t0 || !t1 || !h0) return fail(MPCI_ERR_ARG, "NULL argument");
   if (!(atol >= 0.0) || !(rtol >= 0.0) || atol + rtol <= 0.0 ||
       max_attempts < 1)
@@ -729,7 +891,7 @@ extern "C" int mpci_prop_adaptive(mpci_prop *pr, const double *t0,
   if (track < 0 || track >= pr->sys->N)
     return fail(MPCI_ERR_ARG, "track index out of range");
   pr->track = track;
-  std::vector<Ctl> c(pr->P);
+  c.assign(pr->P, Ctl{});
   std::memset(c.data(), 0, sizeof(Ctl) * pr->P);
   for (int p = 0; p < pr->P; ++p) {
     if (!(h0[p] > 0.0)) return fail(MPCI_ERR_ARG, "h0 must be > 0");
@@ -746,8 +908,18 @@ extern "C" int mpci_prop_adaptive(mpci_prop *pr, const double *t0,
     x.last = (x.t + x.h >= x.t_end);
     x.hc = x.last ? (x.t_end - x.t) : x.h;
   }
-  std::vector<std::vector<LogEnt>> acc(pr->P);
-  TRY(run_controlled(pr, c, acc));
+  acc.assign(pr->P, {});
+  return run_controlled(pr, c, acc);
+}
+
+extern "C" int mpci_prop_adaptive(mpci_prop *pr, const double *t0,
+                                  const double *t1, const double *h0,
+                                  double atol, double rtol, int max_attempts,
+                                  int track, mpci_stats *stats, double *log,
+                                  int log_cap) {
+  std::vector<Ctl> c;
+  std::vector<std::vector<LogEnt>> acc;
+  TRY(run_adaptive(pr, t0, t1, h0, atol, rtol, max_attempts, track, c, acc));
   int rc = MPCI_OK;
   for (int p = 0; p < pr->P; ++p) {
     if (stats) {
@@ -770,14 +942,13 @@ extern "C" int mpci_prop_adaptive(mpci_prop *pr, const double *t0,
   return rc;
 }
 
-extern "C" int mpci_prop_schedule(mpci_prop *pr, const double *t0, int n,
-                                  const double *hs) {
-  if (!pr || !t0 || (n > 0 && !hs) || n < 0)
-    return fail(MPCI_ERR_ARG, "bad argument");
-  if (n == 0) return MPCI_OK;
-  TRY(pr->hs.alloc((size_t)pr->P * n));
-  HIPCHK(hipMemcpy(pr->hs.p, hs, sizeof(double) * pr->P * n,
-                   hipMemcpyHostToDevice));
+// Schedule replay with per-pulse step counts n[p] (<= stride); hs is
+// [P][stride].  Used directly by mpci_prop_schedule and by the adjoint's
+// segment recompute.
+static int run_schedule(mpci_prop *pr, const double *t0, const int *n,
+                        int stride, const double *hs) {
+  TRY(pr->hs.alloc((size_t)pr->P * std::max(stride, 1)));
+  TRY(h2d(pr, pr->hs.p, hs, sizeof(double) * pr->P * stride));
   drop_graph(pr);  // hs moved
   std::vector<Ctl> c(pr->P);
   std::memset(c.data(), 0, sizeof(Ctl) * pr->P);
@@ -786,13 +957,280 @@ extern "C" int mpci_prop_schedule(mpci_prop *pr, const double *t0, int n,
     x.mode = kModeSchedule;
     x.fmode = kFieldPulse;
     x.t = t0[p];
-    x.hc = hs[(size_t)p * n];
+    x.nsteps = n[p];
+    x.hstride = stride;
+    x.max_attempts = n[p];
+    x.done = n[p] <= 0;
+    x.hc = n[p] > 0 ? hs[(size_t)p * stride] : 0.0;
     x.h = x.hc;
-    x.nsteps = n;
-    x.max_attempts = n;
   }
   std::vector<std::vector<LogEnt>> acc(pr->P);
   return run_controlled(pr, c, acc);
+}
+
+extern "C" int mpci_prop_schedule(mpci_prop *pr, const double *t0, int n,
+                                  const double *hs) {
+  if (!pr || !t0 || (n > 0 && !hs) || n < 0)
+    return fail(MPCI_ERR_ARG, "bad argument");
+  if (n == 0) return MPCI_OK;
+  std::vector<int> nn(pr->P, n);
+  return run_schedule(pr, t0, nn.data(), n, hs);
+}
+
+// ---------------------------------------------------------------------
+// Discrete adjoint (extension).
+// ---------------------------------------------------------------------
+namespace {
+// Fehlberg 7(8) nodes c_s, as in the device tableau (stage times t + c_s h).
+const double kHostC[kStages] = {0, 2.0 / 27.0, 1.0 / 9.0, 1.0 / 6.0,
+                                5.0 / 12.0, 0.5, 5.0 / 6.0, 1.0 / 6.0,
+                                2.0 / 3.0, 1.0 / 3.0, 1.0, 0, 1.0};
+
+// dF/d env for the piecewise-linear pulse at time t (the derivative of the
+// device's field_at with respect to the two samples it interpolates).
+void pwl_sensitivity(const PulseDev &pl, double t, int *k, double *w0,
+                     double *w1) {
+  *k = -1;
+  if (!(t >= 0.0 && t <= pl.t_end) || pl.n_env < 2) return;
+  const double dx = pl.t_end / (pl.n_env - 1);
+  const double u = t / dx;
+  int kk = (int)std::floor(u);
+  if (kk > pl.n_env - 2) kk = pl.n_env - 2;
+  if (kk < 0) kk = 0;
+  const double f = u - kk;
+  const double sv = pl.scale * std::sin(pl.w * t + pl.cepd);
+  *k = kk;
+  *w0 = (1.0 - f) * sv;
+  *w1 = f * sv;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now() - t0)
+      .count();
+}
+}  // namespace
+
+extern "C" int mpci_prop_adjoint(mpci_prop *pr, const double *t0,
+                                 const double *t1, const double *h0,
+                                 double atol, double rtol, int max_steps,
+                                 const double *W, int ckpt_every, double *J,
+                                 double *grad, int grad_stride, int *nsteps,
+                                 double *hlog, int hlog_cap,
+                                 mpci_adjoint_stats *stats) {
+  if (!pr || !t0 || !t1 || !h0 || !W || !J || !grad || max_steps < 1 ||
+      ckpt_every < 1)
+    return fail(MPCI_ERR_ARG, "mpci_prop_adjoint: bad argument");
+  const int P = pr->P, N = pr->sys->N;
+  const long long NP = pr->NP;
+  for (int p = 0; p < P; ++p) {
+    if (pr->hpulses[p].kind != kPwl)
+      return fail(MPCI_ERR_ARG, "the adjoint needs piecewise-linear pulses "
+                                "(gradients are taken w.r.t. their samples)");
+    if (pr->hpulses[p].n_env > grad_stride)
+      return fail(MPCI_ERR_ARG, "grad_stride smaller than n_env");
+  }
+  const int K = ckpt_every;
+  const int ckcap = max_steps / K + 2;
+  DevBuf<double2> ckpt, xs_store, xn_store, mu, lam, lamx, xfinal;
+  DevBuf<double> nrm_store, tn_tab, hn_tab, gdev, red2, dotv, Wd;
+  TRY(ckpt.alloc((size_t)ckcap * NP));
+  TRY(xfinal.alloc(NP));
+  TRY(Wd.alloc(N));
+  TRY(h2d(pr, Wd.p, W, sizeof(double) * N));
+  HIPCHK(hipMemcpyAsync(ckpt.p, pr->X.p, sizeof(double2) * NP,
+                        hipMemcpyDeviceToDevice, pr->stream));
+  const bool graph_was = pr->use_graph;
+  pr->use_graph = false;
+  drop_graph(pr);
+  struct Restore {
+    mpci_prop *pr;
+    bool g;
+    ~Restore() {
+      pr->store = StepStore{};
+      pr->use_graph = g;
+      drop_graph(pr);
+    }
+  } restore{pr, graph_was};
+
+  // ---- forward: adaptive, checkpoint every K accepted steps ----------
+  HIPCHK(hipStreamSynchronize(pr->stream));
+  auto tf = std::chrono::steady_clock::now();
+  pr->store = StepStore{};
+  pr->store.ckpt = ckpt.p;
+  pr->store.ckpt_every = K;
+  pr->store.ckpt_cap = ckcap;
+  std::vector<Ctl> c;
+  std::vector<std::vector<LogEnt>> acc;
+  TRY(run_adaptive(pr, t0, t1, h0, atol, rtol, max_steps * 8 + 64, 0, c,
+                   acc));
+  HIPCHK(hipStreamSynchronize(pr->stream));
+  const double fwd_ms = ms_since(tf);
+  std::vector<int> nst(P);
+  int nmax = 0;
+  for (int p = 0; p < P; ++p) {
+    if (c[p].status)
+      return fail(MPCI_ERR_LIMIT, "adjoint forward: attempt limit reached");
+    nst[p] = (int)acc[p].size();
+    if (nst[p] > max_steps)
+      return fail(MPCI_ERR_LIMIT, "adjoint forward took more than max_steps "
+                                  "accepted steps");
+    nmax = std::max(nmax, nst[p]);
+    if (nsteps) nsteps[p] = nst[p];
+    if (hlog)
+      for (int k = 0; k < nst[p] && k < hlog_cap; ++k)
+        hlog[(size_t)p * hlog_cap + k] = acc[p][k].h;
+  }
+  auto t_before = [&](int p, int n) {  // time at the start of step n
+    return n == 0 ? t0[p] : acc[p][n - 1].t;
+  };
+  HIPCHK(hipMemcpyAsync(xfinal.p, pr->X.p, sizeof(double2) * NP,
+                        hipMemcpyDeviceToDevice, pr->stream));
+
+  // ---- objective and its gradient w.r.t. the final state ---------------
+  auto tb = std::chrono::steady_clock::now();
+  TRY(lam.alloc(NP));
+  TRY(lamx.alloc(NP));
+  {
+    const unsigned nb = (unsigned)((NP + 255) / 256);
+    hipLaunchKernelGGL(k_adj_seed, dim3(nb), dim3(256), 0, pr->stream,
+                       pr->X.p, Wd.p, N, P, lam.p);
+    HIPCHK(hipGetLastError());
+    TRY(red2.alloc((size_t)pr->nblk * P));
+    TRY(dotv.alloc(P));
+    // J_p = <x, W x>: reuse the dot kernels on (x, x * W) = (x, lam) / 2
+    hipLaunchKernelGGL(k_dot_partial, dim3(pr->nblk), dim3(256), 0,
+                       pr->stream, pr->X.p, lam.p, N, P, red2.p);
+    hipLaunchKernelGGL(k_reduce_final, dim3(P), dim3(256), 0, pr->stream,
+                       red2.p, pr->nblk, P, dotv.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> jj(P);
+    HIPCHK(hipStreamSynchronize(pr->stream));
+    TRY(d2h(pr, jj.data(), dotv.p, sizeof(double) * P));
+    for (int p = 0; p < P; ++p) J[p] = 0.5 * jj[p];
+  }
+
+  // ---- backward, one checkpoint segment at a time ----------------------
+  TRY(xs_store.alloc((size_t)K * (kStages - 1) * NP));
+  TRY(xn_store.alloc((size_t)(K + 1) * NP));
+  TRY(nrm_store.alloc((size_t)K * P));
+  TRY(mu.alloc((size_t)kStages * NP));
+  TRY(tn_tab.alloc((size_t)K * P));
+  TRY(hn_tab.alloc((size_t)K * P));
+  TRY(gdev.alloc((size_t)K * kStages * P));
+  for (int p = 0; p < P; ++p)
+    for (int k = 0; k < grad_stride; ++k) grad[(size_t)p * grad_stride + k] = 0;
+
+  const int nseg = (nmax + K - 1) / K;
+  std::vector<double> hs((size_t)P * K), tn((size_t)K * P), hn((size_t)K * P),
+      g((size_t)K * kStages * P), ts(P);
+  std::vector<int> nloc(P);
+  for (int j = nseg - 1; j >= 0; --j) {
+    int kmax = 0;
+    for (int p = 0; p < P; ++p) {
+      nloc[p] = std::max(0, std::min(K, nst[p] - j * K));
+      kmax = std::max(kmax, nloc[p]);
+    }
+    if (kmax == 0) continue;
+    // recompute the segment from checkpoint j, storing what the sweep needs
+    HIPCHK(hipMemcpyAsync(pr->X.p, ckpt.p + (size_t)j * NP,
+                          sizeof(double2) * NP, hipMemcpyDeviceToDevice,
+                          pr->stream));
+    HIPCHK(hipMemcpyAsync(xn_store.p, pr->X.p, sizeof(double2) * NP,
+                          hipMemcpyDeviceToDevice, pr->stream));
+    std::fill(hs.begin(), hs.end(), 0.0);
+    std::fill(hn.begin(), hn.end(), 0.0);
+    std::fill(tn.begin(), tn.end(), 0.0);
+    for (int p = 0; p < P; ++p) {
+      ts[p] = t_before(p, std::min(j * K, std::max(nst[p] - 1, 0)));
+      for (int k = 0; k < nloc[p]; ++k) {
+        const int n = j * K + k;
+        hs[(size_t)p * K + k] = acc[p][n].h;
+        hn[(size_t)k * P + p] = acc[p][n].h;
+        tn[(size_t)k * P + p] = t_before(p, n);
+      }
+    }
+    pr->store = StepStore{};
+    pr->store.xs = xs_store.p;
+    pr->store.xn = xn_store.p;
+    pr->store.nrm = nrm_store.p;
+    TRY(run_schedule(pr, ts.data(), nloc.data(), K, hs.data()));
+    pr->store = StepStore{};
+    TRY(h2d(pr, tn_tab.p, tn.data(), sizeof(double) * K * P));
+    TRY(h2d(pr, hn_tab.p, hn.data(), sizeof(double) * K * P));
+    // coupling kernels skip finished pulses: mark all live for the sweep
+    std::vector<Ctl> live(P);
+    std::memset(live.data(), 0, sizeof(Ctl) * P);
+    for (auto &x : live) x.mode = kModeRhs;
+    TRY(write_ctl(pr, live));
+    for (int k = kmax - 1; k >= 0; --k) {
+      AdjArgs a{};
+      a.part = pr->part.p;
+      a.E = pr->sys->E.p;
+      a.pulses = pr->pulses.p;
+      a.env = pr->env.p;
+      a.mu = mu.p;
+      a.lam = lam.p;
+      a.lamx = lamx.p;
+      a.x1 = xn_store.p + (size_t)(k + 1) * NP;
+      a.xn = xn_store.p + (size_t)k * NP;
+      a.xs = xs_store.p + (size_t)k * (kStages - 1) * NP;
+      a.dot = dotv.p;
+      a.nrm = nrm_store.p + (size_t)k * P;
+      a.tn = tn_tab.p + (size_t)k * P;
+      a.hn = hn_tab.p + (size_t)k * P;
+      a.red = red2.p;
+      a.NP = NP;
+      hipLaunchKernelGGL(k_dot_partial, dim3(pr->nblk), dim3(256), 0,
+                         pr->stream, a.x1, lam.p, N, P, red2.p);
+      hipLaunchKernelGGL(k_reduce_final, dim3(P), dim3(256), 0, pr->stream,
+                         red2.p, pr->nblk, P, dotv.p);
+      hipLaunchKernelGGL(k_adj_init, dim3(pr->nblk), dim3(256), 0,
+                         pr->stream, a, pr->desc);
+      HIPCHK(hipGetLastError());
+      for (int st = kStages - 1; st >= 0; --st) {
+        TRY(enqueue_tile(pr, mu.p + (size_t)st * NP, st));
+        a.s = st;
+        hipLaunchKernelGGL(k_adj_stage, dim3(pr->nblk), dim3(256), 0,
+                           pr->stream, a, pr->desc);
+        hipLaunchKernelGGL(k_reduce_final, dim3(P), dim3(256), 0, pr->stream,
+                           red2.p, pr->nblk, P,
+                           gdev.p + ((size_t)k * kStages + st) * P);
+        HIPCHK(hipGetLastError());
+      }
+      std::swap(lam.p, lamx.p);
+    }
+    HIPCHK(hipStreamSynchronize(pr->stream));
+    TRY(d2h(pr, g.data(), gdev.p, sizeof(double) * K * kStages * P));
+    for (int p = 0; p < P; ++p)
+      for (int k = 0; k < nloc[p]; ++k)
+        for (int st = 0; st < kStages; ++st) {
+          const double gv = g[((size_t)k * kStages + st) * P + p];
+          const double h = hn[(size_t)k * P + p];
+          int kk;
+          double w0, w1;
+          pwl_sensitivity(pr->hpulses[p], tn[(size_t)k * P + p] + kHostC[st] * h,
+                          &kk, &w0, &w1);
+          if (kk >= 0) {
+            grad[(size_t)p * grad_stride + kk] += gv * w0;
+            grad[(size_t)p * grad_stride + kk + 1] += gv * w1;
+          }
+        }
+  }
+  const double bwd_ms = ms_since(tb);
+  // leave the propagator at the forward's final state, all pulses finished
+  HIPCHK(hipMemcpyAsync(pr->X.p, xfinal.p, sizeof(double2) * NP,
+                        hipMemcpyDeviceToDevice, pr->stream));
+  HIPCHK(hipStreamSynchronize(pr->stream));
+  for (auto &x : c) x.done = 1;
+  TRY(write_ctl(pr, c));
+  if (stats) {
+    stats->steps_max = nmax;
+    stats->segments = nseg;
+    stats->fwd_ms = fwd_ms;
+    stats->bwd_ms = bwd_ms;
+  }
+  return MPCI_OK;
 }
 
 extern "C" int mpci_prop_populations(mpci_prop *pr, int nobs, const double *W,
@@ -802,13 +1240,12 @@ extern "C" int mpci_prop_populations(mpci_prop *pr, int nobs, const double *W,
   DevBuf<double> dW, dout;
   TRY(dW.alloc((size_t)nobs * N));
   TRY(dout.alloc((size_t)nobs * pr->P));
-  HIPCHK(hipMemcpy(dW.p, W, sizeof(double) * nobs * N, hipMemcpyHostToDevice));
+  TRY(h2d(pr, dW.p, W, sizeof(double) * nobs * N));
   hipLaunchKernelGGL(k_weighted_pop, dim3(pr->P, nobs), dim3(256), 0,
                      pr->stream, pr->X.p, dW.p, N, pr->P, dout.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(pr->stream));
-  HIPCHK(hipMemcpy(out, dout.p, sizeof(double) * nobs * pr->P,
-                   hipMemcpyDeviceToHost));
+  TRY(d2h(pr, out, dout.p, sizeof(double) * nobs * pr->P));
   return MPCI_OK;
 }
 
@@ -829,10 +1266,24 @@ extern "C" int mpci_prop_set_instrument(mpci_prop *pr, int enable) {
 
 extern "C" int mpci_prop_kernel_time(mpci_prop *pr, double *total_ms,
                                      long long *launches,
-                                     long long *bytes_per_launch) {
+                                     long long *bytes_per_launch,
+                                     long long *flops_per_launch) {
   if (!pr) return fail(MPCI_ERR_ARG, "NULL argument");
   if (total_ms) *total_ms = pr->kernel_ms;
   if (launches) *launches = pr->kernel_launches;
+  // algorithmic work of one coupling evaluation: every D_L element streamed
+  // once (GEMV form), or 2 products x 2 flops x 2P real columns (GEMM form)
   if (bytes_per_launch) *bytes_per_launch = 8LL * pr->sys->S;
+  if (flops_per_launch) *flops_per_launch = 8LL * pr->P * pr->sys->S;
+  return MPCI_OK;
+}
+
+extern "C" int mpci_prop_gemm_config(mpci_prop *pr, int *wm, int *wn, int *ks,
+                                     double *ms) {
+  if (!pr) return fail(MPCI_ERR_ARG, "NULL argument");
+  if (wm) *wm = pr->gm_wm;
+  if (wn) *wn = pr->gm_wn;
+  if (ks) *ks = pr->gm_ks;
+  if (ms) *ms = pr->gemm_ms;
   return MPCI_OK;
 }

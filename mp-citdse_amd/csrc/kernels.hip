@@ -78,13 +78,15 @@ __device__ double field_at(const PulseDev &p, const double *env, double t) {
 }
 
 // ---------------------------------------------------------------------
-// One-time dipole re-layout into GEMV tiles.
-// Tile (rb, cb) of pair L holds element (i, j, t, e) at
-// ((i * 4 + j) * 512 + t) * 2 + e, which is D[rb*256 + (t&15) + 16 i]
-// [cb*256 + 2((t>>4) + 32 j) + e] or 0 outside the block.
+// One-time dipole re-layout into the GEMV store.  Row group i of every
+// tile is stored next to row group i of all other tiles (the workgroups
+// that stream them run concurrently, so HBM sees one contiguous front):
+// element (i, job, j, t, e) sits at (((i * njobs + job) * 4 + j) * 512 + t)
+// * 2 + e and holds D[rb*256 + (t&15) + 16 i][cb*256 + 2((t>>4) + 32 j) + e]
+// of the job's tile (rb, cb), or 0 outside the block.
 // ---------------------------------------------------------------------
 __global__ void k_relayout_gemv(const double *__restrict__ D, int sL, int sL1,
-                                int ncb, long long nelem,
+                                int ncb, long long nelem, int job0, int njobs,
                                 double *__restrict__ out) {
   long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nelem) return;
@@ -98,7 +100,11 @@ __global__ void k_relayout_gemv(const double *__restrict__ D, int sL, int sL1,
   int j = ij % kGvColPairIter, i = ij / kGvColPairIter;
   int row = rb * kGvTR + (t & 15) + 16 * i;
   int col = cb * kGvTC + 2 * ((t >> 4) + 32 * j) + el;
-  out[e] = (row < sL && col < sL1) ? D[(long long)row * sL1 + col] : 0.0;
+  const long long job = job0 + tile;
+  const long long dst =
+      ((((long long)i * njobs + job) * kGvColPairIter + j) * kGvThreads + t) *
+          2 + el;
+  out[dst] = (row < sL && col < sL1) ? D[(long long)row * sL1 + col] : 0.0;
 }
 
 // ---------------------------------------------------------------------
@@ -147,18 +153,23 @@ __global__ __launch_bounds__(kGvThreads) void k_gemv_tile(
   // zero-padded tile store and is fetched by the j = 0 load anyway.  Such a
   // value only meets exact zeros (xs0 and xc are zero outside the block) or
   // feeds partials that are never stored, so it needs no masking.
-  const double2 *Dt = reinterpret_cast<const double2 *>(Dg + jb.doff);
+  // D is streamed once, so its loads are non-temporal.
+  typedef double dv2 __attribute__((ext_vector_type(2)));
+  const dv2 *Dt = reinterpret_cast<const dv2 *>(Dg) +
+                  (long long)blockIdx.x * kGvColPairIter * kGvThreads + t;
+  const long long gstride = (long long)gridDim.x * kGvColPairIter * kGvThreads;
   const int nI = (nrows + 15) >> 4;  // uniform over the workgroup
   auto load_group = [&](int i, double2 (&v)[kGvColPairIter]) {
     const bool rv = (tr + 16 * i) < nrows;
-    const double2 *base = Dt + i * kGvColPairIter * kGvThreads + t;
+    const dv2 *base = Dt + i * gstride;
 #pragma unroll
     for (int j = 0; j < kGvColPairIter; ++j) {
       int o = (rv && cv[j]) ? j * kGvThreads : 0;
       // opaque to the optimiser: keeps four independent vector loads
       // instead of a branch that reuses the j = 0 load
       asm volatile("" : "+v"(o));
-      v[j] = base[o];
+      const dv2 d = __builtin_nontemporal_load(base + o);
+      v[j] = make_double2(d.x, d.y);
     }
   };
   double2 cur[kGvColPairIter], nxt[kGvColPairIter];
@@ -233,6 +244,173 @@ __global__ __launch_bounds__(kGvThreads) void k_gemv_tile(
 }
 
 // ---------------------------------------------------------------------
+// Batched coupling on FP64 MFMA.  Workgroup = WM x WN waves, each wave a
+// 32 x 32 block of C as 2 x 2 v_mfma_f64_16x16x4 tiles.  Operand layout of
+// the instruction (checked on gfx950): A[i][k] in lane i + 16k, B[k][j] in
+// lane j + 16k, C[4 r + l / 16][l % 16] in register r of lane l.  K is staged through
+// LDS in chunks of 32 (A stored k-major so fragment reads are
+// conflict-free); the next chunk's global loads are issued before the
+// current chunk's MFMAs.
+// ---------------------------------------------------------------------
+typedef double dmf4 __attribute__((ext_vector_type(4)));
+
+template <int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_coupling(
+    const GemmJob *__restrict__ jobs, const double *__restrict__ Dr,
+    const long long *__restrict__ dr_off, const double *__restrict__ X,
+    double *__restrict__ part, SysDesc d, const Ctl *__restrict__ ctl) {
+  constexpr int TM = 32 * WM, TN = 32 * WN, TK = kGmTK;
+  constexpr int NTH = 64 * WM * WN;
+  constexpr int LDA = TM + 2, LDB = TN + 2;  // padded LDS rows
+  constexpr int NA = (TM * TK + NTH - 1) / NTH;
+  constexpr int NB = (TK * TN + NTH - 1) / NTH;
+  __shared__ double As[TK * LDA];
+  __shared__ double Bs[TK * LDB];
+  __shared__ int live;
+
+  const GemmJob jb = jobs[blockIdx.x];
+  const int t = threadIdx.x;
+  const int P2 = 2 * d.P;
+  // skip the panel when every pulse it covers has finished
+  if (t == 0) live = 0;
+  __syncthreads();
+  {
+    const int p0 = jb.n0 >> 1, p1 = min(d.P, (jb.n0 + TN + 1) >> 1);
+    for (int p = p0 + t; p < p1; p += NTH)
+      if (!ctl[p].done) live = 1;
+  }
+  __syncthreads();
+  if (!live) return;
+
+  const int L = jb.L;
+  const int sL = d.off[L + 1] - d.off[L];
+  const int sL1 = d.off[L + 2] - d.off[L + 1];
+  const int Mdim = jb.dir == 0 ? sL : sL1;
+  const double *D = Dr + dr_off[L];
+  const double *Xk = X + (long long)(jb.dir == 0 ? d.off[L + 1] : d.off[L]) * P2;
+
+  const int lane = t & 63, w = t >> 6;
+  const int wm = w % WM, wn = w / WM;
+  dmf4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = dmf4{0.0, 0.0, 0.0, 0.0};
+
+  double ra[NA], rb[NB];
+  auto load_chunk = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int idx = t + u * NTH;
+      int m, k;
+      if (jb.dir == 0) {  // D rows = m: contiguous in k
+        m = idx / TK;
+        k = idx % TK;
+      } else {            // D rows = k: contiguous in m
+        k = idx / TM;
+        m = idx % TM;
+      }
+      const int gm = jb.m0 + m, gk = k0 + k;
+      double v = 0.0;
+      if (idx < TM * TK && gm < Mdim && gk < jb.k1)
+        v = jb.dir == 0 ? D[(long long)gm * sL1 + gk]
+                        : D[(long long)gk * sL1 + gm];
+      ra[u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int idx = t + u * NTH;
+      const int k = idx / TN, n = idx % TN;
+      const int gk = k0 + k, gn = jb.n0 + n;
+      double v = 0.0;
+      if (idx < TK * TN && gk < jb.k1 && gn < P2)
+        v = Xk[(long long)gk * P2 + gn];
+      rb[u] = v;
+    }
+  };
+  auto store_chunk = [&]() {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int idx = t + u * NTH;
+      if (idx < TM * TK) {
+        int m, k;
+        if (jb.dir == 0) {
+          m = idx / TK;
+          k = idx % TK;
+        } else {
+          k = idx / TM;
+          m = idx % TM;
+        }
+        As[k * LDA + m] = ra[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int idx = t + u * NTH;
+      if (idx < TK * TN) Bs[(idx / TN) * LDB + idx % TN] = rb[u];
+    }
+  };
+
+  const int li = lane & 15, lk = lane >> 4;
+  load_chunk(jb.k0);
+  for (int k0 = jb.k0; k0 < jb.k1; k0 += TK) {
+    __syncthreads();  // previous chunk fully consumed
+    store_chunk();
+    __syncthreads();
+    if (k0 + TK < jb.k1) load_chunk(k0 + TK);  // uniform branch
+#pragma unroll
+    for (int ks = 0; ks < TK / 4; ++ks) {
+      const int kr = ks * 4 + lk;
+      double a[2], b[2];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+        a[bi] = As[kr * LDA + wm * 32 + bi * 16 + li];
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+        b[bj] = Bs[kr * LDB + wn * 32 + bj * 16 + li];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              a[bi], b[bj], acc[bi][bj], 0, 0, 0);
+    }
+  }
+
+  // write the panel into partial slot q of this pair and direction
+  const long long base =
+      jb.dir == 0 ? d.row_part_off[L] : d.col_part_off[L];  // double2 units
+  double *out = part + 2 * base + (long long)jb.q * Mdim * P2;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = jb.m0 + wm * 32 + bi * 16 + 4 * r + lk;
+        const int n = jb.n0 + wn * 32 + bj * 16 + li;
+        if (m < Mdim && n < P2) out[(long long)m * P2 + n] = acc[bi][bj][r];
+      }
+}
+
+template __global__ void k_gemm_coupling<2, 2>(const GemmJob *, const double *,
+                                               const long long *,
+                                               const double *, double *,
+                                               SysDesc, const Ctl *);
+template __global__ void k_gemm_coupling<2, 4>(const GemmJob *, const double *,
+                                               const long long *,
+                                               const double *, double *,
+                                               SysDesc, const Ctl *);
+template __global__ void k_gemm_coupling<4, 2>(const GemmJob *, const double *,
+                                               const long long *,
+                                               const double *, double *,
+                                               SysDesc, const Ctl *);
+template __global__ void k_gemm_coupling<4, 4>(const GemmJob *, const double *,
+                                               const long long *,
+                                               const double *, double *,
+                                               SysDesc, const Ctl *);
+
+// ---------------------------------------------------------------------
 // Stage kernel.  Thread (row, p) with rows_per_block = 256 / P.
 // ---------------------------------------------------------------------
 __device__ inline int block_of(const SysDesc &d, int row) {
@@ -287,7 +465,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
       const double e = a.E[row];
       double F;
       if (c.fmode == kFieldTable)
-        F = a.Ftab[(long long)p * c.nsteps + c.step];
+        F = a.Ftab[(long long)p * c.hstride + c.step];
       else if (c.fmode == kFieldPulse)
         F = field_at(a.pulses[p], a.env, c.t + cC[s] * c.hc);
       else
@@ -324,6 +502,9 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
           vx += co * k.x;
           vy += co * k.y;
           a.Xs[idx] = make_double2(vx, vy);
+          if (a.xs_store)
+            a.xs_store[((long long)c.step * (kStages - 1) + s) * NP + idx] =
+                make_double2(vx, vy);
         } else {
           double yx = 1.0 * x.x, yy = 1.0 * x.y, ex = 0.0, ey = 0.0;
           double2 kj[kStages];
@@ -448,7 +629,7 @@ __global__ __launch_bounds__(256) void k_finalize(Ctl *__restrict__ ctl,
     if (c.step >= c.nsteps)
       c.done = 1;
     else if (c.mode == kModeSchedule)
-      c.hc = hs[(long long)p * c.nsteps + c.step];
+      c.hc = hs[(long long)p * c.hstride + c.step];
   }
   ctl[p] = c;
 }
@@ -458,7 +639,7 @@ __global__ __launch_bounds__(256) void k_apply(double2 *__restrict__ X,
                                                const Ctl *__restrict__ ctl,
                                                LogEnt *__restrict__ log,
                                                int logcap, int track, int N,
-                                               int P) {
+                                               int P, StepStore st) {
   const int rpb = 256 / P;
   const int t = threadIdx.x;
   const int lr = t / P, p = t - lr * P;
@@ -467,6 +648,7 @@ __global__ __launch_bounds__(256) void k_apply(double2 *__restrict__ X,
   const Ctl &c = ctl[p];
   if (!c.accept) return;
   const long long idx = (long long)row * P + p;
+  const long long NP = (long long)N * P;
   const double2 y = Y[idx];
   const double2 xn = make_double2(y.x / c.nrm, y.y / c.nrm);
   X[idx] = xn;
@@ -476,6 +658,158 @@ __global__ __launch_bounds__(256) void k_apply(double2 *__restrict__ X,
     le.h = c.hacc;
     le.pop = xn.x * xn.x + xn.y * xn.y;
     log[(long long)p * logcap + c.nlog - 1] = le;
+  }
+  if (st.xn) st.xn[(long long)c.step * NP + idx] = xn;
+  if (st.nrm && row == 0) st.nrm[(long long)(c.step - 1) * P + p] = c.nrm;
+  if (st.ckpt && c.mode == kModeAdaptive && c.accepted % st.ckpt_every == 0) {
+    const int slot = c.accepted / st.ckpt_every;
+    if (slot < st.ckpt_cap) st.ckpt[(long long)slot * NP + idx] = xn;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Discrete adjoint of one RKF78 step with renormalisation.
+// Forward: X_s = x + h sum_j a_sj k_j, k_s = A(F_s) X_s, y = x + h sum b_s k_s,
+// x1 = y / ||y||.  Backward, with A(F)^T = -A(F) in the real representation
+// (the generator is anti-Hermitian) and C the coupling at unit field:
+//   lam_y = (lam - x1 <x1, lam>) / ||y||,  mu_s = h b_s lam_y,  lamx = lam_y
+//   for s = 12..0:  nu = -A(F_s) mu_s,  dJ/dF_s = -<C mu_s, X_s>,
+//                   lamx += nu,  mu_j += h a_sj nu (j < s).
+// ---------------------------------------------------------------------
+// lam = dJ/dx for J = sum_i W_i |x_i|^2
+__global__ __launch_bounds__(256) void k_adj_seed(const double2 *__restrict__ X,
+                                                  const double *__restrict__ W,
+                                                  int N, int P,
+                                                  double2 *__restrict__ lam) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long long)N * P) return;
+  const double w = W[e / P];
+  const double2 x = X[e];
+  lam[e] = make_double2(2.0 * w * x.x, 2.0 * w * x.y);
+}
+
+// per-block partials of <a, b> for every pulse (rows_per_block = 256 / P)
+__global__ __launch_bounds__(256) void k_dot_partial(
+    const double2 *__restrict__ a, const double2 *__restrict__ b, int N, int P,
+    double *__restrict__ red) {
+  const int rpb = 256 / P;
+  const int t = threadIdx.x;
+  const int lr = t / P, p = t - lr * P;
+  const int row = blockIdx.x * rpb + lr;
+  double v = 0.0;
+  if (lr < rpb && row < N) {
+    const long long idx = (long long)row * P + p;
+    v = a[idx].x * b[idx].x + a[idx].y * b[idx].y;
+  }
+  __shared__ double sv[256];
+  sv[t] = v;
+  __syncthreads();
+  if (t < P) {
+    double acc = 0.0;
+    for (int r = 0; r < rpb; ++r) acc += sv[r * P + t];
+    red[(long long)blockIdx.x * P + t] = acc;
+  }
+}
+
+// out[p] = sum over blocks of red[b * P + p], fixed order; one WG per pulse
+__global__ __launch_bounds__(256) void k_reduce_final(
+    const double *__restrict__ red, int nblk, int P, double *__restrict__ out) {
+  const int p = blockIdx.x, t = threadIdx.x;
+  double acc = 0.0;
+  for (int b = t; b < nblk; b += 256) acc += red[(long long)b * P + p];
+  __shared__ double sv[256];
+  sv[t] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) sv[t] += sv[t + w];
+    __syncthreads();
+  }
+  if (t == 0) out[p] = sv[0];
+}
+
+__global__ __launch_bounds__(256) void k_adj_init(AdjArgs a, SysDesc d) {
+  const int P = d.P;
+  const int rpb = 256 / P;
+  const int t = threadIdx.x;
+  const int lr = t / P, p = t - lr * P;
+  const int row = blockIdx.x * rpb + lr;
+  if (lr >= rpb || row >= d.N) return;
+  const long long idx = (long long)row * P + p;
+  const double2 lam = a.lam[idx];
+  const double h = a.hn[p];
+  if (!(h > 0.0)) {  // pulse not active at this step: pass through
+    a.lamx[idx] = lam;
+    return;
+  }
+  const double2 x1 = a.x1[idx];
+  const double dt = a.dot[p], nr = a.nrm[p];
+  const double2 ly =
+      make_double2((lam.x - x1.x * dt) / nr, (lam.y - x1.y * dt) / nr);
+  a.lamx[idx] = ly;
+#pragma unroll
+  for (int s = 0; s < kStages; ++s) {
+    const double cb = h * cB[s];
+    a.mu[s * a.NP + idx] = make_double2(cb * ly.x, cb * ly.y);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adj_stage(AdjArgs a, SysDesc d) {
+  const int P = d.P;
+  const int rpb = 256 / P;
+  const int t = threadIdx.x;
+  const int lr = t / P, p = t - lr * P;
+  const int row = blockIdx.x * rpb + lr;
+  const int s = a.s;
+  double gpart = 0.0;
+  if (lr < rpb && row < d.N && a.hn[p] > 0.0) {
+    const int L = block_of(d, row);
+    const int i = row - d.off[L];
+    const int sL = d.off[L + 1] - d.off[L];
+    const long long idx = (long long)row * P + p;
+    const long long NP = a.NP;
+    double2 cs = make_double2(0.0, 0.0), rs = make_double2(0.0, 0.0);
+    if (L > 0)
+      cs = sum_partials(a.part + d.col_part_off[L - 1], d.ncol_part[L - 1],
+                        (long long)sL * P, (long long)i * P + p);
+    if (L < d.nb - 1)
+      rs = sum_partials(a.part + d.row_part_off[L], d.nrow_part[L],
+                        (long long)sL * P, (long long)i * P + p);
+    const double h = a.hn[p];
+    const double F = field_at(a.pulses[p], a.env, a.tn[p] + cC[s] * h);
+    const double2 mu = a.mu[s * NP + idx];
+    const double e = a.E[row];
+    double2 cm;  // C mu: the coupling at unit field
+    if (d.gauge == 0) {
+      cm = make_double2(cs.x - rs.x, cs.y - rs.y);
+    } else {
+      cm = make_double2(cs.y + rs.y, -(cs.x + rs.x));
+    }
+    // nu = -(A0 mu + F C mu), A0 = -iE
+    const double2 nu =
+        make_double2(-(e * mu.y + F * cm.x), -(-e * mu.x + F * cm.y));
+    const double2 xs = (s == 0) ? a.xn[idx] : a.xs[(long long)(s - 1) * NP + idx];
+    gpart = -(cm.x * xs.x + cm.y * xs.y);
+    double2 lx = a.lamx[idx];
+    lx.x += nu.x;
+    lx.y += nu.y;
+    a.lamx[idx] = lx;
+    for (int j = 0; j < s; ++j) {
+      const double co = cA[s][j];
+      if (co != 0.0) {
+        double2 m = a.mu[j * NP + idx];
+        m.x += (h * co) * nu.x;
+        m.y += (h * co) * nu.y;
+        a.mu[j * NP + idx] = m;
+      }
+    }
+  }
+  __shared__ double sv[256];
+  sv[t] = gpart;
+  __syncthreads();
+  if (t < P) {
+    double acc = 0.0;
+    for (int r = 0; r < rpb; ++r) acc += sv[r * P + t];
+    a.red[(long long)blockIdx.x * P + t] = acc;
   }
 }
 

@@ -17,21 +17,58 @@ struct StageArgs {
   const PulseDev *pulses;
   const double *env;     // envelope samples of PWL pulses
   const double *Ftab;    // frozen per-step fields (P x nsteps)
+  double2 *xs_store;     // optional: stage inputs of every step
   long long NP;
   int s;                 // stage 0..12
   int pad;
 };
 
+// Backward (adjoint) sweep of one step for all pulses.
+struct AdjArgs {
+  const double2 *part;   // coupling partials of mu_s
+  const double *E;
+  const PulseDev *pulses;
+  const double *env;
+  double2 *mu;           // 13 stage adjoints, stride NP
+  double2 *lam;          // adjoint of the state after the step (input)
+  double2 *lamx;         // adjoint of the state before the step (output)
+  const double2 *x1;     // normalised state after the step
+  const double2 *xn;     // state before the step (stage 0 input)
+  const double2 *xs;     // stage inputs 1..12 of the step, stride NP
+  const double *dot;     // <x1, lam> per pulse
+  const double *nrm;     // ||y|| per pulse
+  const double *tn;      // step start time per pulse
+  const double *hn;      // step size per pulse (<= 0: pulse inactive)
+  double *red;           // per-block partials of dJ/dF_s
+  long long NP;
+  int s;
+  int pad;
+};
+
 __global__ void k_relayout_gemv(const double *D, int sL, int sL1, int ncb,
-                                long long nelem, double *out);
+                                long long nelem, int job0, int njobs,
+                                double *out);
 __global__ void k_gemv_tile(const TileJob *jobs, const double *Dg,
                             const double2 *X, double2 *part, SysDesc d,
                             const Ctl *ctl);
+template <int WM, int WN>
+__global__ void k_gemm_coupling(const GemmJob *jobs, const double *Dr,
+                                const long long *dr_off, const double *X,
+                                double *part, SysDesc d, const Ctl *ctl);
 __global__ void k_stage(StageArgs a, SysDesc d);
 __global__ void k_finalize(Ctl *ctl, const double *red, int nblk, int P,
                            const double *hs);
 __global__ void k_apply(double2 *X, const double2 *Y, const Ctl *ctl,
-                        LogEnt *log, int logcap, int track, int N, int P);
+                        LogEnt *log, int logcap, int track, int N, int P,
+                        StepStore st);
+__global__ void k_adj_seed(const double2 *X, const double *W, int N, int P,
+                           double2 *lam);
+__global__ void k_dot_partial(const double2 *a, const double2 *b, int N, int P,
+                              double *red);
+__global__ void k_reduce_final(const double *red, int nblk, int P,
+                               double *out);
+__global__ void k_adj_init(AdjArgs a, SysDesc d);
+__global__ void k_adj_stage(AdjArgs a, SysDesc d);
 __global__ void k_weighted_pop(const double2 *X, const double *W, int N, int P,
                                double *out);
 

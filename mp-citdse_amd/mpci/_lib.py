@@ -40,6 +40,11 @@ class MpciStats(C.Structure):
                 ("t", C.c_double), ("h", C.c_double)]
 
 
+class MpciAdjointStats(C.Structure):
+    _fields_ = [("steps_max", C.c_int), ("segments", C.c_int),
+                ("fwd_ms", C.c_double), ("bwd_ms", C.c_double)]
+
+
 class MpciError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"mpci error {code}: {msg}")
@@ -78,10 +83,16 @@ _SIGS = [
                                      C.POINTER(MpciStats), _dp, C.c_int]),
     ("mpci_prop_schedule", C.c_int, [_vp, _dp, C.c_int, _dp]),
     ("mpci_prop_populations", C.c_int, [_vp, C.c_int, _dp, _dp]),
+    ("mpci_prop_adjoint", C.c_int, [_vp, _dp, _dp, _dp, C.c_double,
+                                    C.c_double, C.c_int, _dp, C.c_int, _dp,
+                                    _dp, C.c_int, _ip, _dp, C.c_int,
+                                    C.POINTER(MpciAdjointStats)]),
     ("mpci_prop_set_graph", C.c_int, [_vp, C.c_int]),
     ("mpci_prop_set_instrument", C.c_int, [_vp, C.c_int]),
     ("mpci_prop_kernel_time", C.c_int, [_vp, _dp, C.POINTER(C.c_longlong),
+                                        C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_longlong)]),
+    ("mpci_prop_gemm_config", C.c_int, [_vp, _ip, _ip, _ip, _dp]),
 ]
 
 # Every symbol the header declares; tests check the library exports them.

@@ -181,6 +181,32 @@ class Propagator:
         check(_lib.tdse_lib().mpci_prop_schedule(self.h, dptr(t0), hs.shape[1],
                                                  dptr(hs)))
 
+    def adjoint(self, t0, t1, h0, W, atol=1e-10, rtol=1e-10, max_steps=100000,
+                ckpt_every=32, hlog_cap=0):
+        """Discrete-adjoint gradient of J_p = sum_k W_k |c_{k,p}(t1)|^2 with
+        respect to each pulse's PWL envelope samples.  Returns (J, grad,
+        nsteps, hlog, stats)."""
+        P = self.P
+        t0 = np.ascontiguousarray(np.broadcast_to(t0, (P,)), dtype=np.float64)
+        t1 = np.ascontiguousarray(np.broadcast_to(t1, (P,)), dtype=np.float64)
+        h0 = np.ascontiguousarray(np.broadcast_to(h0, (P,)), dtype=np.float64)
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        if W.shape != (self.sys.N,):
+            raise ValueError("W must hold N weights")
+        nenv = max(int(p.c.n_env) for p in self._pulses)
+        J = np.empty(P)
+        grad = np.zeros((P, nenv))
+        nst = np.zeros(P, dtype=np.int32)
+        cap = max(hlog_cap, 1)
+        hlog = np.zeros((P, cap))
+        st = _lib.MpciAdjointStats()
+        check(_lib.tdse_lib().mpci_prop_adjoint(
+            self.h, dptr(t0), dptr(t1), dptr(h0), atol, rtol, max_steps,
+            dptr(W), ckpt_every, dptr(J), dptr(grad), nenv,
+            nst.ctypes.data_as(_lib._ip), dptr(hlog) if hlog_cap else None,
+            hlog_cap, C.byref(st)))
+        return J, grad, nst, hlog[:, :hlog_cap], st
+
     def populations(self, W) -> np.ndarray:
         W = np.ascontiguousarray(np.atleast_2d(W), dtype=np.float64)
         out = np.empty((W.shape[0], self.P))
@@ -195,12 +221,24 @@ class Propagator:
         check(_lib.tdse_lib().mpci_prop_set_instrument(self.h, int(on)))
 
     def kernel_time(self):
+        """(total ms, launches, bytes per launch, flops per launch) of the
+        coupling kernel over the instrumented attempts."""
         ms = C.c_double()
         n = C.c_longlong()
         b = C.c_longlong()
+        f = C.c_longlong()
         check(_lib.tdse_lib().mpci_prop_kernel_time(self.h, C.byref(ms),
-                                                    C.byref(n), C.byref(b)))
-        return ms.value, n.value, b.value
+                                                    C.byref(n), C.byref(b),
+                                                    C.byref(f)))
+        return ms.value, n.value, b.value, f.value
+
+    def gemm_config(self):
+        wm, wn, ks = C.c_int(), C.c_int(), C.c_int()
+        ms = C.c_double()
+        check(_lib.tdse_lib().mpci_prop_gemm_config(
+            self.h, C.byref(wm), C.byref(wn), C.byref(ks), C.byref(ms)))
+        return {"wm": wm.value, "wn": wn.value, "ksplit": ks.value,
+                "ms": ms.value}
 
 
 def device_count() -> int:

@@ -1,0 +1,203 @@
+"""End-to-end GPU runs of the `tdse` drop-in (src/tdse/tdse_main.cpp:9-201)
+on reference-format HDF5 inputs written by tdse_gen, against the oracle on
+the same leading blocks: coefficient-file names and cadence (t0, after step
+0, every steps/10, final), the field file (default precision) and the
+population file (16 digits), full-precision final states, restart with
+-s, the adaptive extension, and the 1e file prefixes with a gaussian
+length-gauge pulse."""
+import math
+import os
+import subprocess
+import textwrap
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "mp-citdse_amd", "bin")
+EV = 1.0 / 27.211396181
+IO = (1.0 / 3.509338) * 1.0e-16
+
+
+def yaml_2e(states, gauge="v", shape="sine", cycles=2, extra=""):
+    return textwrap.dedent(f"""\
+        Global_Settings:
+          potential: "he"
+          L_max: {len(states) - 1}
+          gauge: '{gauge}'
+        Basis_Settings:
+          l_max: 7
+        Propagator_Settings:
+          states_in_l: [{",".join(map(str, states))}]
+          dt: 0.1
+          full_precision: true
+        {extra}Field_Parameters:
+          shape: "{shape}"
+          w: 15
+          Io: 1.0e14
+          cepd: 0.0
+          cycles: {cycles}
+        """)
+
+
+def read_state(path, n):
+    v = np.loadtxt(path, comments="#")
+    assert v.shape == (n, 3)
+    x = np.empty(2 * n)
+    x[0::2], x[1::2] = v[:, 1], v[:, 2]
+    return x
+
+
+def run(args, cwd):
+    r = subprocess.run([os.path.join(BIN, "tdse")] + args, cwd=cwd,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r
+
+
+def leading(full, states, seed=20261202):
+    from mpci import synth
+    E, D = synth.system_arrays(full, seed=seed)
+    E = [E[L][:n] for L, n in enumerate(states)]
+    D = [D[L][:states[L], :states[L + 1]] for L in range(len(states) - 1)]
+    return E, D
+
+
+def test_cli_2e_reference_mode(tmp_path, oracle):
+    full, states = [40, 35, 30], [30, 35, 25]
+    subprocess.run([os.path.join(BIN, "tdse_gen"), "-e", "2", "-p", "he",
+                    "-s", ",".join(map(str, full)), "-g", "v"], cwd=tmp_path,
+                   check=True)
+    (tmp_path / "He.yaml").write_text(yaml_2e(states))
+    r = run(["-e", "2", "-f", "He.yaml", "-o", "out"], tmp_path)
+    E, D = leading(full, states)
+    o = oracle.System(E, D, gauge="v")
+    w = 15 * EV
+    tau = 2 * 2.0 * math.pi / w
+    steps = int(tau / 0.1)
+    pulse = oracle.Pulse.sine(1.0e14 * IO, w, 2, gauge="v")
+    x0 = np.zeros(2 * o.N)
+    x0[0] = 1.0
+    xr, tr, popr, fldr = o.prop_fixed(pulse, 0.0, 0.1, steps, x0, track=0)
+    out = tmp_path / "out"
+    # cadence: t0, after step 0, every steps/10, final
+    pr, t, want = steps // 10, 0.0, {"%f" % 0.0}
+    for st in range(steps):
+        t += 0.1
+        if st % pr == 0:
+            want.add("%f" % t)
+    want.add("%f" % t)
+    got = {f[len("he_ct_"):-4] for f in os.listdir(out)
+           if f.startswith("he_ct_")}
+    assert got == want
+    # final state at full precision
+    x = read_state(out / ("he_ct17_%f.dat" % t), o.N)
+    assert np.abs(x - xr).max() < 1e-12
+    # population file: 16 digits, one line per step plus t0
+    pop = np.loadtxt(out / "he_pop10.dat", comments="#")
+    assert pop.shape == (steps + 1, 2)
+    assert np.abs(pop[1:, 1] - popr).max() < 1e-12
+    # field file: default ostream precision, F(t + dt) as the reference
+    lines = (out / "he_field.dat").read_text().splitlines()
+    assert lines[0] == "#time (a.u.), field (A(t) or E(t)) (a.u.)"
+    tt = 0.0
+    for k in range(steps):
+        assert lines[k + 1] == f"{tt + 0.1:.6g} {fldr[k]:.6g}"
+        tt += 0.1
+    # printed fields at the coefficient-file steps
+    printed = [float(v) for v in r.stdout.split("number of cycles:")[1]
+               .split()[1:]]
+    assert len(printed) == len(range(0, steps, pr))
+
+
+def test_cli_restart(tmp_path, oracle):
+    full, states = [30, 30, 30], [30, 30, 30]
+    subprocess.run([os.path.join(BIN, "tdse_gen"), "-e", "2", "-p", "he",
+                    "-s", "30,30,30", "-g", "l"], cwd=tmp_path, check=True)
+    (tmp_path / "He.yaml").write_text(yaml_2e(states, gauge="l"))
+    run(["-e", "2", "-f", "He.yaml", "-o", "out"], tmp_path)
+    ct = sorted(f for f in os.listdir(tmp_path / "out")
+                if f.startswith("he_ct_"))[3]
+    t_start = float(ct[len("he_ct_"):-4])
+    r = run(["-e", "2", "-f", "He.yaml", "-o", "out2", "-s", f"out/{ct}"],
+            tmp_path)
+    assert f"Start time (a.u.): {t_start:g}" in r.stdout
+    E, D = leading(full, states)
+    o = oracle.System(E, D, gauge="l")
+    w = 15 * EV
+    tau = 2 * 2.0 * math.pi / w
+    steps = int((tau - t_start) / 0.1)
+    x0 = read_state(tmp_path / "out" / ct, o.N)  # 6-digit restart values
+    pulse = oracle.Pulse.sine(1.0e14 * IO, w, 2, gauge="l")
+    xr, tr, _, _ = o.prop_fixed(pulse, t_start, 0.1, steps, x0)
+    x = read_state(tmp_path / "out2" / ("he_ct17_%f.dat" % tr), o.N)
+    assert np.abs(x - xr).max() < 1e-12
+
+
+def test_cli_adaptive_extension(tmp_path, oracle):
+    full = states = [26, 31, 22]
+    subprocess.run([os.path.join(BIN, "tdse_gen"), "-e", "2", "-p", "he",
+                    "-s", "26,31,22", "-g", "v"], cwd=tmp_path, check=True)
+    (tmp_path / "He.yaml").write_text(
+        yaml_2e(states, extra="").replace(
+            "  full_precision: true\n",
+            "  full_precision: true\n  mode: \"adaptive\"\n  tol: 1.0e-10\n"))
+    run(["-e", "2", "-f", "He.yaml", "-o", "out"], tmp_path)
+    E, D = leading(full, states)
+    o = oracle.System(E, D, gauge="v")
+    w = 15 * EV
+    tau = 2 * 2.0 * math.pi / w
+    pulse = oracle.Pulse.sine(1.0e14 * IO, w, 2, gauge="v")
+    x = np.zeros(2 * o.N)
+    x[0] = 1.0
+    t, h = 0.0, 0.1
+    for k in range(1, 11):  # ten segments, as the drop-in runs them
+        t1 = tau if k == 10 else 0.0 + k * (tau - 0.0) / 10.0
+        x, st, _ = o.prop_adaptive(pulse, t, t1, h, 1e-10, 1e-10, x)
+        t, h = st.t, st.h
+    xg = read_state(tmp_path / "out" / ("he_ct17_%f.dat" % tau), o.N)
+    pops = lambda v: v[0::2] ** 2 + v[1::2] ** 2
+    assert np.abs(pops(xg) - pops(x)).max() < 1e-8
+    assert np.abs(xg - x).max() < 1e-10
+
+
+def test_cli_1e_gaussian_length(tmp_path, oracle):
+    states = [20, 18, 16, 14]
+    subprocess.run([os.path.join(BIN, "tdse_gen"), "-e", "1", "-p", "h",
+                    "-s", "20,18,16,14", "-g", "l"], cwd=tmp_path, check=True)
+    y = textwrap.dedent("""\
+        Global_Settings:
+          potential: "h"
+          gauge: 'l'
+        Basis_Settings:
+          l_max: 3
+        Propagator_Settings:
+          states_in_l: [20,18,16,14]
+          dt: 0.05
+          full_precision: true
+        Field_Parameters:
+          shape: "gaussian"
+          w: 8.5
+          Io: 1.0e13
+          cepd: 0.0
+          cycles: 3
+        """)
+    (tmp_path / "H.yaml").write_text(y)
+    run(["-e", "1", "-f", "H.yaml", "-o", "out", "-n", "2", "-l", "1"],
+        tmp_path)
+    E, D = leading(states, states)
+    o = oracle.System(E, D, gauge="l")
+    w = 8.5 * EV
+    tau = 3 * 2.0 * math.pi / w
+    steps = int(tau / 0.05)
+    pulse = oracle.Pulse.gauss(1.0e13 * IO, w, tau, 3, gauge="l")
+    x0 = np.zeros(2 * o.N)
+    x0[0] = 1.0
+    track = 20 + 1  # -n 2 -l 1: offs[1] + (2 - 1)
+    xr, tr, popr, _ = o.prop_fixed(pulse, 0.0, 0.05, steps, x0, track=track)
+    x = read_state(tmp_path / "out" / ("h_ct17_%f.dat" % tr), o.N)
+    assert np.abs(x - xr).max() < 1e-12
+    pop = np.loadtxt(tmp_path / "out" / "h_pop21.dat", comments="#")
+    assert np.abs(pop[1:, 1] - popr).max() < 1e-12
