@@ -53,8 +53,12 @@ __constant__ double cC[kStages] = {0, 2.0 / 27.0, 1.0 / 9.0, 1.0 / 6.0,
 // ---------------------------------------------------------------------
 // Laser field, src/tdse/pulse.cpp:34-39 (gauss), 52-56 (sineAA), 93-97
 // (sineEE), plus the piecewise-linear envelope extension.
+// Always inlined: hipcc (ROCm 7.2) emitted this as a call from inside a
+// divergent branch and copied the saved thread id under the branch's exec
+// mask, so lanes outside the branch lost threadIdx.x across the call.
 // ---------------------------------------------------------------------
-__device__ double field_at(const PulseDev &p, const double *env, double t) {
+__device__ __forceinline__ double field_at(const PulseDev &p,
+                                           const double *env, double t) {
   if (p.kind == kSine2) {
     double sinWt = sin(p.a * t);
     return (double)(t >= 0.0 && t <= M_PI / p.a) * p.Eo * sinWt * sinWt *
@@ -413,13 +417,13 @@ template __global__ void k_gemm_coupling<4, 4>(const GemmJob *, const double *,
 // ---------------------------------------------------------------------
 // Stage kernel.  Thread (row, p) with rows_per_block = 256 / P.
 // ---------------------------------------------------------------------
-__device__ inline int block_of(const SysDesc &d, int row) {
+__device__ __forceinline__ int block_of(const SysDesc &d, int row) {
   int L = 0;
   while (L + 1 < d.nb && row >= d.off[L + 1]) ++L;
   return L;
 }
 
-__device__ inline double2 sum_partials(const double2 *base, int n,
+__device__ __forceinline__ double2 sum_partials(const double2 *base, int n,
                                        long long stride, long long at) {
   double2 acc = make_double2(0.0, 0.0);
   for (int q0 = 0; q0 < n; q0 += 8) {
