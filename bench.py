@@ -12,9 +12,11 @@ coupling-kernel launch on the propagator's stream, which gives the average
 duration of the dominant kernel (k_gemv_tile) for the roofline figure.
 
 Extra figures on the same line: the reference mode (fixed dt = 0.1, field
-frozen per step, as src/tdse/tdse.cpp:186-204) and the CPU baseline (the
-oracle's restatement of the reference with MKL cblas_dgemv, timed on the
-host cores over a bounded sample).
+frozen per step, as src/tdse/tdse.cpp:186-204), the batch of 64 pulses on
+FP64 MFMA, the uneven multi-instance batch (32 H-shaped + 32 He pulses),
+the discrete adjoint, and the CPU baseline (the oracle's restatement of the
+reference with MKL cblas_dgemv, timed on the host cores over a bounded
+sample).
 """
 from __future__ import annotations
 
@@ -123,6 +125,67 @@ def run_batch(mpci, synth, sysg, P=64, attempts=200):
             "avg_launch_us": avg_ms * 1e3}
 
 
+S_H = [998] * 21
+H_SEED_OFFSET = 1
+
+
+def run_multi(mpci, synth, sysg, P=32, attempts=120):
+    """Uneven multi-instance batch: P pulses on an H-shaped synthetic system
+    (21 blocks of 998 states, E ~ U[-0.5, 5]) and P pulses on the He
+    system, pulses drawn as for the batch line, propagated concurrently by
+    mpci_multi_adaptive for a bounded number of attempts; the same two
+    batches are then run one after the other for comparison."""
+    import numpy as np
+    E, D = synth.system_arrays(S_H, seed=synth.HE_SEED + H_SEED_OFFSET,
+                               e_lo=-0.5, e_hi=5.0)
+    sysh = mpci.System(E, D, gauge="v")
+    del E, D
+    members = []
+    for g, seed in ((sysh, synth.HE_SEED + 5), (sysg, synth.HE_SEED + 6)):
+        w, E0, cyc = synth.pulse_batch(P, seed=seed)
+        t1 = np.array([mpci.period(w[p], int(cyc[p])) for p in range(P)])
+        prop = mpci.Propagator(g, P=P)
+        for p in range(P):
+            prop.set_pulse(mpci.Pulse.sine(E0[p] ** 2, w[p], int(cyc[p])), p)
+        members.append((g, prop, t1))
+
+    def reset():
+        for g, prop, _ in members:
+            x0 = synth.ground_state(g.N)
+            for p in range(P):
+                prop.set_state(x0, p)
+
+    props = [m[1] for m in members]
+    t1s = [m[2] for m in members]
+    reset()
+    mpci.multi_adaptive(props, [0.0, 0.0], t1s, [0.05, 0.05], TOL, TOL,
+                        max_attempts=5, allow_limit=True)
+    reset()
+    t0 = time.perf_counter()
+    st = mpci.multi_adaptive(props, [0.0, 0.0], t1s, [0.05, 0.05], TOL, TOL,
+                             max_attempts=attempts, allow_limit=True)
+    wall = time.perf_counter() - t0
+    reset()
+    seq = []
+    for prop, t1 in zip(props, t1s):
+        t0 = time.perf_counter()
+        prop.prop_adaptive(0.0, t1, 0.05, TOL, TOL, max_attempts=attempts,
+                           allow_limit=True)
+        seq.append(time.perf_counter() - t0)
+    flops = 0.0
+    for (g, _, _), sm in zip(members, st):
+        flops += sum(x.attempts for x in sm) * 13 * 8.0 * g.dipole_elements
+    acc = [sum(x.accepted for x in sm) for sm in st]
+    tf = flops / wall / 1e12
+    return {"P_h": P, "P_he": P, "attempts": attempts,
+            "h_blocks": len(S_H), "h_states_per_block": S_H[0],
+            "accepted_h": acc[0], "accepted_he": acc[1],
+            "pulse_steps_per_s": sum(acc) / wall, "wall_s": wall,
+            "sequential_wall_s": sum(seq),
+            "overlap_speedup": sum(seq) / wall,
+            "mfma_tflops_wall": tf, "mfma_frac_wall": tf / 78.6}
+
+
 def run_adjoint(mpci, synth, sysg, B=16, K=256, cycles=20, w=0.7):
     """Discrete-adjoint gradients of the final ground-state population with
     respect to K piecewise-linear envelope samples (uniform in [0, 0.04]
@@ -181,6 +244,20 @@ def cpu_baseline(E, D, budget_s=12.0):
                       f"(tol 1e-10) with the oracle, {blas}, {wall:.1f} s"}
 
 
+def copy_calibration(n_bytes=1 << 30, reps=3):
+    """A plain elementwise copy (torch.mul by 1.0, out=) of n_bytes: each
+    launch reads and writes exactly n_bytes, which scales the hardware byte
+    counters for the kernels measured in the same profiled run."""
+    import torch
+    x = torch.ones(n_bytes // 8, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    return {"bytes_read": n_bytes, "bytes_written": n_bytes, "launches": reps,
+            "kernel": "torch elementwise mul (copy)"}
+
+
 def traffic_from_profiles():
     p = os.path.join(ROOT, "profiles", "k_gemv_tile_he4x4000_traffic.json")
     try:
@@ -198,6 +275,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-multi", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed run, launch a plain copy kernel "
+                         "of 1 GiB read + 1 GiB write (scale reference for "
+                         "the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build propagates on a "
@@ -255,12 +337,19 @@ def main():
             b["mfma_frac"] = b["mfma_tflops"] / 78.6
         out["batch64"] = {k: (round(v, 4) if isinstance(v, float) else v)
                           for k, v in b.items()}
+    if not args.no_multi:
+        log("bench: multi-instance 32 H (21 x 998) + 32 He ...")
+        mres = run_multi(mpci, synth, sysg)
+        out["multi_h32_he32"] = {k: (round(v, 4) if isinstance(v, float)
+                                     else v) for k, v in mres.items()}
     if not args.no_adjoint:
         log("bench: adjoint (16 envelopes x 256 samples) ...")
         a = run_adjoint(mpci, synth, sysg)
         out["adjoint"] = {k: (round(v, 4) if isinstance(v, float) else v)
                           for k, v in a.items()}
         out["adjoint_over_forward"] = round(a["adjoint_over_forward"], 4)
+    if args.calibrate_copy:
+        out["copy_calibration"] = copy_calibration()
     if not args.no_cpu_baseline:
         log("bench: CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(E, D)

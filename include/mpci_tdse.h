@@ -136,6 +136,19 @@ int mpci_prop_adaptive(mpci_prop *prop, const double *t0, const double *t1,
                        int max_attempts, int track, mpci_stats *stats,
                        double *log, int log_cap);
 
+/* Multi-instance adaptive propagation (extension; the reference runs one
+ * system per process, src/tdse/tdse_main.cpp:169-194).  Runs
+ * mpci_prop_adaptive(props[i], t0[i], t1[i], h0[i], atol, rtol,
+ * max_attempts, track 0, stats[i], no log) for n distinct propagators, which
+ * may hold different systems and batch sizes, concurrently on one device:
+ * each member keeps its own stream and its next chunk is launched as soon
+ * as its previous one drains.  Results equal n separate calls.  stats[i] may
+ * be NULL.  Returns MPCI_ERR_LIMIT when any pulse hit max_attempts. */
+int mpci_multi_adaptive(int n, mpci_prop *const *props,
+                        const double *const *t0, const double *const *t1,
+                        const double *const *h0, double atol, double rtol,
+                        int max_attempts, mpci_stats *const *stats);
+
 /* Replays the adaptive map over n given steps hs[p * n + k] from t0[p]
  * (no controller): used for frozen-schedule finite differences. */
 int mpci_prop_schedule(mpci_prop *prop, const double *t0, int n,

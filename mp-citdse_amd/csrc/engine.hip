@@ -14,6 +14,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mpci_tdse.h"
@@ -849,29 +850,33 @@ extern "C" int mpci_prop_fixed(mpci_prop *pr, double t0, double dt, int steps,
   return MPCI_OK;
 }
 
+// Attempts to launch in the next chunk (0: every pulse has finished).
+static int plan_chunk(const mpci_prop *pr, const std::vector<Ctl> &c) {
+  int est = 0;
+  bool any = false;
+  for (auto &x : c) {
+    if (x.done) continue;
+    any = true;
+    int e;
+    if (x.mode == kModeAdaptive) {
+      // Launch about half of the steps still expected: the proposed h
+      // only grows by 4.5x per step, so the chunk rarely overshoots and
+      // the number of host synchronisations stays logarithmic.
+      double r = 0.5 * (x.t_end - x.t) / std::max(x.h, 1e-300);
+      e = (int)std::min(1e6, std::ceil(r));
+    } else {
+      e = x.nsteps - x.step;
+    }
+    est = std::max(est, e);
+  }
+  if (!any) return 0;
+  return std::max(1, std::min(est, pr->logcap));
+}
+
 static int run_controlled(mpci_prop *pr, std::vector<Ctl> &c,
                           std::vector<std::vector<LogEnt>> &acc) {
   TRY(write_ctl(pr, c));
-  for (;;) {
-    int est = 0;
-    bool any = false;
-    for (auto &x : c) {
-      if (x.done) continue;
-      any = true;
-      int e;
-      if (x.mode == kModeAdaptive) {
-        // Launch about half of the steps still expected: the proposed h
-        // only grows by 4.5x per step, so the chunk rarely overshoots and
-        // the number of host synchronisations stays logarithmic.
-        double r = 0.5 * (x.t_end - x.t) / std::max(x.h, 1e-300);
-        e = (int)std::min(1e6, std::ceil(r));
-      } else {
-        e = x.nsteps - x.step;
-      }
-      est = std::max(est, e);
-    }
-    if (!any) break;
-    const int n = std::max(1, std::min(est, pr->logcap));
+  for (int n; (n = plan_chunk(pr, c)) > 0;) {
     TRY(run_attempts(pr, n));
     TRY(read_ctl(pr, c));
     TRY(drain_log(pr, c, acc));
@@ -879,10 +884,10 @@ static int run_controlled(mpci_prop *pr, std::vector<Ctl> &c,
   return MPCI_OK;
 }
 
-static int run_adaptive(mpci_prop *pr, const double *t0, const double *t1,
-                        const double *h0, double atol, double rtol,
-                        int max_attempts, int track, std::vector<Ctl> &c,
-                        std::vector<std::vector<LogEnt>> &acc) {
+static int setup_adaptive(mpci_prop *pr, const double *t0, const double *t1,
+                          const double *h0, double atol, double rtol,
+                          int max_attempts, int track, std::vector<Ctl> &c,
+                          std::vector<std::vector<LogEnt>> &acc) {
   if (!pr || !t0 || !t1 || !h0) return fail(MPCI_ERR_ARG, "NULL argument");
   if (!(atol >= 0.0) || !(rtol >= 0.0) || atol + rtol <= 0.0 ||
       max_attempts < 1)
@@ -909,7 +914,26 @@ static int run_adaptive(mpci_prop *pr, const double *t0, const double *t1,
     x.hc = x.last ? (x.t_end - x.t) : x.h;
   }
   acc.assign(pr->P, {});
+  return MPCI_OK;
+}
+
+static int run_adaptive(mpci_prop *pr, const double *t0, const double *t1,
+                        const double *h0, double atol, double rtol,
+                        int max_attempts, int track, std::vector<Ctl> &c,
+                        std::vector<std::vector<LogEnt>> &acc) {
+  TRY(setup_adaptive(pr, t0, t1, h0, atol, rtol, max_attempts, track, c, acc));
   return run_controlled(pr, c, acc);
+}
+
+static void fill_stats(const std::vector<Ctl> &c, mpci_stats *stats) {
+  for (size_t p = 0; p < c.size(); ++p) {
+    stats[p].accepted = c[p].accepted;
+    stats[p].rejected = c[p].rejected;
+    stats[p].attempts = c[p].attempts;
+    stats[p].status = c[p].status ? MPCI_ERR_LIMIT : MPCI_OK;
+    stats[p].t = c[p].t;
+    stats[p].h = c[p].h;
+  }
 }
 
 extern "C" int mpci_prop_adaptive(mpci_prop *pr, const double *t0,
@@ -921,15 +945,8 @@ extern "C" int mpci_prop_adaptive(mpci_prop *pr, const double *t0,
   std::vector<std::vector<LogEnt>> acc;
   TRY(run_adaptive(pr, t0, t1, h0, atol, rtol, max_attempts, track, c, acc));
   int rc = MPCI_OK;
+  if (stats) fill_stats(c, stats);
   for (int p = 0; p < pr->P; ++p) {
-    if (stats) {
-      stats[p].accepted = c[p].accepted;
-      stats[p].rejected = c[p].rejected;
-      stats[p].attempts = c[p].attempts;
-      stats[p].status = c[p].status ? MPCI_ERR_LIMIT : MPCI_OK;
-      stats[p].t = c[p].t;
-      stats[p].h = c[p].h;
-    }
     if (c[p].status) rc = fail(MPCI_ERR_LIMIT, "adaptive: max_attempts hit");
     if (log)
       for (int k = 0; k < (int)acc[p].size() && k < log_cap; ++k) {
@@ -938,6 +955,83 @@ extern "C" int mpci_prop_adaptive(mpci_prop *pr, const double *t0,
         e[1] = acc[p][k].h;
         e[2] = acc[p][k].pop;
       }
+  }
+  return rc;
+}
+
+// Several propagators (instances: different systems, different P) stepped
+// concurrently.  Each member runs on its own stream; whenever a member's
+// chunk has drained, its log is collected and its next chunk is enqueued at
+// once, so a small instance never waits for a large one and the device
+// always has the work of every unfinished member queued.
+extern "C" int mpci_multi_adaptive(int n, mpci_prop *const *props,
+                                   const double *const *t0,
+                                   const double *const *t1,
+                                   const double *const *h0, double atol,
+                                   double rtol, int max_attempts,
+                                   mpci_stats *const *stats) {
+  if (n < 1 || !props || !t0 || !t1 || !h0)
+    return fail(MPCI_ERR_ARG, "mpci_multi_adaptive: bad argument");
+  for (int i = 0; i < n; ++i) {
+    if (!props[i] || !t0[i] || !t1[i] || !h0[i])
+      return fail(MPCI_ERR_ARG, "mpci_multi_adaptive: NULL member");
+    for (int j = 0; j < i; ++j)
+      if (props[j] == props[i])
+        return fail(MPCI_ERR_ARG, "mpci_multi_adaptive: a propagator is "
+                                  "listed twice");
+  }
+  struct Member {
+    mpci_prop *pr;
+    std::vector<Ctl> c;
+    std::vector<std::vector<LogEnt>> acc;
+    bool busy = false, finished = false;
+  };
+  std::vector<Member> m(n);
+  for (int i = 0; i < n; ++i) {
+    m[i].pr = props[i];
+    TRY(setup_adaptive(props[i], t0[i], t1[i], h0[i], atol, rtol,
+                       max_attempts, 0, m[i].c, m[i].acc));
+    TRY(write_ctl(props[i], m[i].c));
+  }
+  for (;;) {
+    int live = 0;
+    for (auto &x : m) {
+      if (x.finished || x.busy) {
+        live += !x.finished;
+        continue;
+      }
+      const int k = plan_chunk(x.pr, x.c);
+      if (k == 0) {
+        x.finished = true;
+        continue;
+      }
+      TRY(run_attempts(x.pr, k));
+      x.busy = true;
+      ++live;
+    }
+    if (live == 0) break;
+    // wait for any member's stream to drain
+    for (bool got = false; !got;) {
+      for (auto &x : m) {
+        if (!x.busy) continue;
+        const hipError_t q = hipStreamQuery(x.pr->stream);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess)
+          return fail(MPCI_ERR_HIP, std::string("hipStreamQuery: ") +
+                                        hipGetErrorString(q));
+        TRY(read_ctl(x.pr, x.c));
+        TRY(drain_log(x.pr, x.c, x.acc));
+        x.busy = false;
+        got = true;
+      }
+      if (!got) std::this_thread::yield();
+    }
+  }
+  int rc = MPCI_OK;
+  for (int i = 0; i < n; ++i) {
+    if (stats && stats[i]) fill_stats(m[i].c, stats[i]);
+    for (auto &x : m[i].c)
+      if (x.status) rc = fail(MPCI_ERR_LIMIT, "adaptive: max_attempts hit");
   }
   return rc;
 }

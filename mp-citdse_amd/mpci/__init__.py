@@ -6,4 +6,4 @@ that interface.
 """
 from ._lib import MpciError, EXPORTED  # noqa: F401
 from .engine import (Propagator, Pulse, System, device_count,  # noqa: F401
-                     period, to_au)
+                     multi_adaptive, period, to_au)

@@ -81,6 +81,11 @@ _SIGS = [
     ("mpci_prop_adaptive", C.c_int, [_vp, _dp, _dp, _dp, C.c_double,
                                      C.c_double, C.c_int, C.c_int,
                                      C.POINTER(MpciStats), _dp, C.c_int]),
+    ("mpci_multi_adaptive", C.c_int, [C.c_int, C.POINTER(_vp),
+                                      C.POINTER(_dp), C.POINTER(_dp),
+                                      C.POINTER(_dp), C.c_double, C.c_double,
+                                      C.c_int,
+                                      C.POINTER(C.POINTER(MpciStats))]),
     ("mpci_prop_schedule", C.c_int, [_vp, _dp, C.c_int, _dp]),
     ("mpci_prop_populations", C.c_int, [_vp, C.c_int, _dp, _dp]),
     ("mpci_prop_adjoint", C.c_int, [_vp, _dp, _dp, _dp, C.c_double,

@@ -241,6 +241,35 @@ class Propagator:
                 "ms": ms.value}
 
 
+def multi_adaptive(props, t0, t1, h0, atol, rtol, max_attempts=10_000_000,
+                   allow_limit=False):
+    """Adaptive propagation of several propagators (different systems and
+    batch sizes) concurrently on one device; equal to separate
+    Propagator.prop_adaptive calls.  t0, t1, h0: one scalar or P-vector per
+    member.  Returns one list of per-pulse stats per member."""
+    n = len(props)
+    keep, st = [], []
+
+    def vec(v, P):
+        a = np.ascontiguousarray(np.broadcast_to(v, (P,)), dtype=np.float64)
+        keep.append(a)
+        return dptr(a)
+
+    hs = (_lib._vp * n)(*[p.h.value for p in props])
+    a0 = (_lib._dp * n)(*[vec(t0[i], props[i].P) for i in range(n)])
+    a1 = (_lib._dp * n)(*[vec(t1[i], props[i].P) for i in range(n)])
+    ah = (_lib._dp * n)(*[vec(h0[i], props[i].P) for i in range(n)])
+    for p in props:
+        st.append((MpciStats * p.P)())
+    sp = (C.POINTER(MpciStats) * n)(
+        *[C.cast(s, C.POINTER(MpciStats)) for s in st])
+    rc = _lib.tdse_lib().mpci_multi_adaptive(n, hs, a0, a1, ah, atol, rtol,
+                                             max_attempts, sp)
+    if not (allow_limit and rc == _lib.MPCI_ERR_LIMIT):
+        check(rc)
+    return [list(s) for s in st]
+
+
 def device_count() -> int:
     n = C.c_int()
     check(_lib.tdse_lib().mpci_device_count(C.byref(n)))
