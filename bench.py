@@ -39,6 +39,16 @@ W_AU, E0_AU, CYCLES = 0.7, 0.03, 10
 TOL = 1e-10
 
 
+GRAPH = True
+
+
+def make_prop(mpci, sysg, P=1):
+    prop = mpci.Propagator(sysg, P=P)
+    if not GRAPH:
+        prop.set_graph(False)
+    return prop
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -49,7 +59,7 @@ def he_system(mpci, synth):
 
 
 def run_gpu(args, mpci, synth, sysg):
-    prop = mpci.Propagator(sysg)
+    prop = make_prop(mpci, sysg)
     pulse = mpci.Pulse.sine(E0_AU ** 2, W_AU, CYCLES, gauge="v")
     prop.set_pulse(pulse)
     tau = mpci.period(W_AU, CYCLES)
@@ -76,7 +86,7 @@ def run_gpu(args, mpci, synth, sysg):
 
 def run_reference_mode(mpci, synth, sysg, steps=200, dt=0.1):
     """Fixed dt, field frozen at t + dt per step (src/tdse/tdse.cpp:187)."""
-    prop = mpci.Propagator(sysg)
+    prop = make_prop(mpci, sysg)
     pulse = mpci.Pulse.sine(E0_AU ** 2, W_AU, CYCLES, gauge="v")
     import numpy as np
     F, t = np.empty((1, steps)), 0.0
@@ -98,7 +108,7 @@ def run_batch(mpci, synth, sysg, P=64, attempts=200):
     bounded number of lockstep attempts from t = 0."""
     import numpy as np
     w, E0, cyc = synth.pulse_batch(P)
-    prop = mpci.Propagator(sysg, P=P)
+    prop = make_prop(mpci, sysg, P=P)
     x0 = synth.ground_state(sysg.N)
     t1 = np.array([mpci.period(w[p], int(cyc[p])) for p in range(P)])
     for p in range(P):
@@ -144,7 +154,7 @@ def run_multi(mpci, synth, sysg, P=32, attempts=120):
     for g, seed in ((sysh, synth.HE_SEED + 5), (sysg, synth.HE_SEED + 6)):
         w, E0, cyc = synth.pulse_batch(P, seed=seed)
         t1 = np.array([mpci.period(w[p], int(cyc[p])) for p in range(P)])
-        prop = mpci.Propagator(g, P=P)
+        prop = make_prop(mpci, g, P=P)
         for p in range(P):
             prop.set_pulse(mpci.Pulse.sine(E0[p] ** 2, w[p], int(cyc[p])), p)
         members.append((g, prop, t1))
@@ -192,7 +202,7 @@ def run_adjoint(mpci, synth, sysg, B=16, K=256, cycles=20, w=0.7):
     a.u.) for a batch of B envelopes, checkpoints every 32 steps."""
     import numpy as np
     T = mpci.period(w, cycles)
-    prop = mpci.Propagator(sysg, P=B)
+    prop = make_prop(mpci, sysg, P=B)
     x0 = synth.ground_state(sysg.N)
     W = np.zeros(sysg.N)
     W[0] = 1.0
@@ -276,6 +286,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-multi", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every attempt eagerly instead of replaying "
+                         "the captured step graph (for tools that cannot "
+                         "follow graph launches)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed run, launch a plain copy kernel "
                          "of 1 GiB read + 1 GiB write (scale reference for "
@@ -286,6 +300,8 @@ def main():
               "single MI355X (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    global GRAPH
+    GRAPH = not args.no_graph
     import mpci
     from mpci import synth
     log("bench: building He system (4 x 4000) ...")

@@ -103,7 +103,9 @@ int mpci_system_info(const mpci_system *sys, int *N, long long *bytes);
 /* ---- propagator ------------------------------------------------------ */
 /* P pulses (1 <= P <= 256).  P = 1 streams D once per RHS through the fused
  * GEMV kernel (HBM-bound); P >= 2 runs the coupling as FP64 MFMA GEMMs with
- * the tile shape picked by an on-device autotuner at creation. */
+ * the tile shape picked by an on-device autotuner at the first creation for
+ * this system and P; later propagators of the same system and P reuse it,
+ * so they sum in the same order and give bitwise-equal results. */
 int mpci_prop_create(mpci_system *sys, int P, mpci_prop **out);
 int mpci_prop_destroy(mpci_prop *prop);
 /* ct: 2N doubles for pulse p (tdse_main.cpp:148-160 initial state). */

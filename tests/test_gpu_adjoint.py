@@ -124,8 +124,13 @@ def test_batched_adjoint_uneven_pulses(mpci, oracle):
         single.set_pulse(mpci.Pulse.pwl(envs[p], w, Ts[p]))
         J1, g1, n1, _, _ = single.adjoint(0.0, Ts[p], 0.05, W, ckpt_every=9)
         assert n1[0] == nst[p]
-        assert abs(J1[0] - J[p]) < 1e-12
-        assert rel(grad[p], g1[0]) < 1e-10
+        # the batch couples on MFMA, the single pulse on the GEMV kernel:
+        # their sums differ in the last bits, the accepted step sizes then
+        # differ by ~1e-7 relative (the embedded error cancels ~10 digits),
+        # the final states by up to ~1e-10 (the adaptive-mode bar) and the
+        # gradient of each discrete map by ~1e-10 relative
+        assert abs(J1[0] - J[p]) < 1e-10
+        assert rel(grad[p], g1[0]) < 1e-8
         fd = gpu_fd(mpci, g, envs[p], w, Ts[p], "v", hlog[p, :nst[p]], W, x0)
         assert rel(grad[p], fd) < 1e-6
 
