@@ -112,14 +112,57 @@ __global__ void k_relayout_gemv(const double *__restrict__ D, int sL, int sL1,
 }
 
 // ---------------------------------------------------------------------
+// Cross-lane sums on the VALU (DPP and the gfx950 lane swaps), no LDS.
+// ---------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF,
+                                             0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF,
+                                             0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 16 lanes of a row: quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror.  Each step adds two operands, which commute
+// exactly, so every lane of the row ends with the same bits.
+__device__ __forceinline__ double sum16(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x140>(v);
+  return v;
+}
+
+// v[l] + v[l ^ 16]: v_permlane16_swap of v with itself leaves the even rows
+// in the first result and the odd rows in the second, so lane l holds both
+// operands; l and l ^ 16 add them in the same order.
+__device__ __forceinline__ double add_xor16(double v) {
+  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  return __hiloint2double((int)h[0], (int)l[0]) +
+         __hiloint2double((int)h[1], (int)l[1]);
+}
+
+// v[l] + v[l ^ 32], likewise with v_permlane32_swap (wave halves).
+__device__ __forceinline__ double add_xor32(double v) {
+  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __hiloint2double((int)h[0], (int)l[0]) +
+         __hiloint2double((int)h[1], (int)l[1]);
+}
+
+// ---------------------------------------------------------------------
 // Fused single-pass coupling over one 256 x 256 tile of D_L.
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(kGvThreads) void k_gemv_tile(
     const TileJob *__restrict__ jobs, const double *__restrict__ Dg,
     const double2 *__restrict__ X, double2 *__restrict__ part, SysDesc d,
     const Ctl *__restrict__ ctl) {
-  if (ctl[0].done) return;
   const TileJob jb = jobs[blockIdx.x];
+  if (ctl[0].done) return;
   const int L = jb.L;
   const int sL = d.off[L + 1] - d.off[L];
   const int sL1 = d.off[L + 2] - d.off[L + 1];
@@ -133,22 +176,9 @@ __global__ __launch_bounds__(kGvThreads) void k_gemv_tile(
   __shared__ double2 xs0[kGvTR];
   __shared__ double2 rowbuf[kGvThreads / 64][kGvTR];
 
-  for (int r = t; r < kGvTR; r += kGvThreads)
-    xs0[r] = (r < nrows) ? x0[row0 + r] : make_double2(0.0, 0.0);
-
-  double2 xc[kGvColPairIter][2];
-  double2 acc[kGvColPairIter][2];
   bool cv[kGvColPairIter];
 #pragma unroll
-  for (int j = 0; j < kGvColPairIter; ++j) {
-    const int c = col0 + 2 * (g + 32 * j);
-    cv[j] = c < sL1;
-    xc[j][0] = (c < sL1) ? x1[c] : make_double2(0.0, 0.0);
-    xc[j][1] = (c + 1 < sL1) ? x1[c + 1] : make_double2(0.0, 0.0);
-    acc[j][0] = make_double2(0.0, 0.0);
-    acc[j][1] = make_double2(0.0, 0.0);
-  }
-  __syncthreads();
+  for (int j = 0; j < kGvColPairIter; ++j) cv[j] = col0 + 2 * (g + 32 * j) < sL1;
 
   // Software-pipelined stream over the tile: the loads of row group i + 1
   // are in flight while row group i is consumed.  Every load is issued
@@ -176,8 +206,25 @@ __global__ __launch_bounds__(kGvThreads) void k_gemv_tile(
       v[j] = make_double2(d.x, d.y);
     }
   };
+  // the first row group's loads go out before the x loads, so the tile's
+  // stream starts one memory latency earlier
   double2 cur[kGvColPairIter], nxt[kGvColPairIter];
   load_group(0, cur);
+
+  double2 xc[kGvColPairIter][2];
+  double2 acc[kGvColPairIter][2];
+#pragma unroll
+  for (int j = 0; j < kGvColPairIter; ++j) {
+    const int c = col0 + 2 * (g + 32 * j);
+    xc[j][0] = (c < sL1) ? x1[c] : make_double2(0.0, 0.0);
+    xc[j][1] = (c + 1 < sL1) ? x1[c + 1] : make_double2(0.0, 0.0);
+    acc[j][0] = make_double2(0.0, 0.0);
+    acc[j][1] = make_double2(0.0, 0.0);
+  }
+  for (int r = t; r < kGvTR; r += kGvThreads)
+    xs0[r] = (r < nrows) ? x0[row0 + r] : make_double2(0.0, 0.0);
+  __syncthreads();
+
   for (int i = 0; i < nI; ++i) {
     // unconditional prefetch (the last one re-reads group nI-1 from cache)
     // keeps the wait counters exact across iterations
@@ -198,10 +245,8 @@ __global__ __launch_bounds__(kGvThreads) void k_gemv_tile(
       acc[j][1].y = fma(dv.y, xr.y, acc[j][1].y);
     }
     // sum the row partial over the 4 column groups held by this wave
-    pre += __shfl_xor(pre, 16);
-    pim += __shfl_xor(pim, 16);
-    pre += __shfl_xor(pre, 32);
-    pim += __shfl_xor(pim, 32);
+    pre = add_xor32(add_xor16(pre));
+    pim = add_xor32(add_xor16(pim));
     if (lane < 16) rowbuf[w][r] = make_double2(pre, pim);
 #pragma unroll
     for (int j = 0; j < kGvColPairIter; ++j) cur[j] = nxt[j];
@@ -212,14 +257,8 @@ __global__ __launch_bounds__(kGvThreads) void k_gemv_tile(
   for (int j = 0; j < kGvColPairIter; ++j) {
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
-      double ax = acc[j][e].x, ay = acc[j][e].y;
-#pragma unroll
-      for (int m = 1; m < 16; m <<= 1) {
-        ax += __shfl_xor(ax, m);
-        ay += __shfl_xor(ay, m);
-      }
-      acc[j][e].x = ax;
-      acc[j][e].y = ay;
+      acc[j][e].x = sum16(acc[j][e].x);
+      acc[j][e].y = sum16(acc[j][e].y);
     }
   }
   if (tr == 0) {
