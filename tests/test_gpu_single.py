@@ -140,6 +140,31 @@ def test_graph_and_eager_agree(mpci, oracle):
     assert np.array_equal(out[0], out[1])
 
 
+def test_instrumented_runs_agree(mpci, oracle):
+    """The instrumented mode (eager launches with an event pair around every
+    coupling kernel) gives the graph result bit for bit and counts 13
+    coupling launches per attempt that did work."""
+    g, _ = build(mpci, oracle, [130, 90, 110], "l", seed=9)
+    x0 = np.zeros(2 * g.N)
+    x0[0] = 1.0
+    pg = mpci.Pulse.sine(0.04 ** 2, 0.7, 3, "l")
+    T = mpci.period(0.7, 3)
+    out = []
+    for instr in (False, True):
+        prop = mpci.Propagator(g)
+        prop.set_instrument(instr)
+        prop.set_state(x0)
+        prop.set_pulse(pg)
+        st, _ = prop.prop_adaptive(0.0, T, 0.1, 1e-10, 1e-10)
+        assert st[0].attempts > 64  # more than one instrumented chunk
+        out.append(prop.get_state())
+        if instr:
+            ms, n, b, f = prop.kernel_time()
+            assert n == 13 * st[0].attempts, (n, st[0].attempts)
+            assert ms > 0 and b == 8 * g.dipole_elements
+    assert np.array_equal(out[0], out[1])
+
+
 def test_full_he_shape_two_steps(mpci, oracle):
     """Config-2 He shape (4 blocks of 4000): two reference-mode steps."""
     g, o = build(mpci, oracle, [4000] * 4, "v", seed=20261202)
