@@ -9,9 +9,10 @@ is accepted steps per second with all inputs resident in HBM.
 
 The timed region replays the captured step graph once per attempt (the
 production mode).  The same attempts are then run once more with a HIP
-event pair around each coupling-kernel launch on the propagator's stream,
-which gives the average duration of the dominant kernel (k_gemv_tile) for
-the roofline figure (events do not time kernels inside graph replays).
+event pair attached to each coupling-kernel dispatch on the propagator's
+stream (hipExtLaunchKernelGGL), which gives the average duration of the
+dominant kernel (k_gemv_tile) for the roofline figure (events do not time
+kernels inside graph replays).
 
 Extra figures on the same line: the reference mode (fixed dt = 0.1, field
 frozen per step, as src/tdse/tdse.cpp:186-204), the batch of 64 pulses on
@@ -77,9 +78,10 @@ def run_gpu(args, mpci, synth, sysg):
     st, _ = prop.prop_adaptive(0.0, tau, 0.05, TOL, TOL,
                                max_attempts=args.steps, allow_limit=True)
     wall = time.perf_counter() - t0
-    # the same attempts again, launched individually with a HIP event pair
-    # around every coupling kernel on the propagator's stream (events do not
-    # time kernels inside graph replays): the dominant kernel's duration
+    # the same attempts again, launched individually, every coupling kernel
+    # carrying a HIP event pair on its own dispatch on the propagator's
+    # stream (events do not time kernels inside graph replays): the dominant
+    # kernel's duration
     prop.set_state(x0)
     prop.set_instrument(True)
     st_i, _ = prop.prop_adaptive(0.0, tau, 0.05, TOL, TOL,
@@ -302,6 +304,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-multi", action="store_true")
+    ap.add_argument("--no-reference-mode", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every attempt eagerly instead of replaying "
                          "the captured step graph (for tools that cannot "
@@ -329,8 +332,10 @@ def main():
         f"{r['wall']:.3f} s")
     avg_us = r["kernel_ms"] / max(r["kernel_launches"], 1) * 1e3
     achieved = r["kernel_bytes"] / (avg_us * 1e-6) / 1e9
-    log("bench: reference mode ...")
-    ref = run_reference_mode(mpci, synth, sysg)
+    ref = None
+    if not args.no_reference_mode:
+        log("bench: reference mode ...")
+        ref = run_reference_mode(mpci, synth, sysg)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -358,9 +363,10 @@ def main():
                      "avg_launch_us": round(avg_us, 3),
                      "bytes_per_launch": r["kernel_bytes"],
                      "launches": r["kernel_launches"]},
-        "reference_mode": {"steps_per_s": round(ref["steps_per_s"], 3),
-                           "dt": ref["dt"], "steps": ref["steps"]},
     }
+    if ref is not None:
+        out["reference_mode"] = {"steps_per_s": round(ref["steps_per_s"], 3),
+                                 "dt": ref["dt"], "steps": ref["steps"]}
     if not args.no_batch:
         log("bench: batch-64 ...")
         b = run_batch(mpci, synth, sysg)

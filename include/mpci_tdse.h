@@ -97,6 +97,10 @@ int mpci_system_create(int nb, const int *s, char gauge,
                        const double *const *E, const double *const *D,
                        mpci_system **out);
 int mpci_system_destroy(mpci_system *sys);
+/* Tuning: number of workgroups of the persistent single-pulse coupling
+ * kernel (each streams every G-th tile of the dipole store); 0 restores the
+ * default of 2 per compute unit.  Results do not depend on it. */
+int mpci_system_set_gemv_grid(mpci_system *sys, int workgroups);
 /* N = sum of s; bytes = device memory held by the system. */
 int mpci_system_info(const mpci_system *sys, int *N, long long *bytes);
 
@@ -183,8 +187,10 @@ int mpci_prop_populations(mpci_prop *prop, int nobs, const double *W,
 /* Launch the 13-stage step as one captured hipGraph (1, default) or as
  * individual launches (0). */
 int mpci_prop_set_graph(mpci_prop *prop, int enable);
-/* Instrumentation: when on, steps are launched individually with HIP
- * events around every coupling kernel, accumulated into
+/* Instrumentation: when on, steps are launched individually and every
+ * coupling kernel carries a pair of HIP events on its own dispatch
+ * (hipExtLaunchKernelGGL), so each interval is that kernel's execution;
+ * the durations of the attempts that did work accumulate into
  * mpci_prop_kernel_time.  Off by default. */
 int mpci_prop_set_instrument(mpci_prop *prop, int enable);
 int mpci_prop_kernel_time(mpci_prop *prop, double *total_ms,

@@ -20,14 +20,15 @@ constexpr int kMaxBlocks = 32;   // L_max <= 31
 constexpr int kStages = 13;      // Fehlberg 7(8)
 
 // ---- fused GEMV tile geometry (single pulse, HBM-bound form) ----------
-// A workgroup of 512 lanes streams one TR x TC tile of a dipole block D_L
-// and produces, in one pass, the row partials of D_L x_{L+1} and the
-// column partials of D_L^T x_L.  Lane t covers rows (t & 15) + 16 i and
+// A workgroup of 512 lanes streams TR x TC tiles of the dipole blocks D_L
+// (persistently, several per workgroup) and produces for each, in one
+// pass, the row partials of D_L x_{L+1} and the column partials of
+// D_L^T x_L.  Lane t covers rows (t & 15) + 16 i and
 // column pairs (t >> 4) + 32 j.
 constexpr int kGvThreads = 512;
-constexpr int kGvRowIter = 16;                 // i
+constexpr int kGvRowIter = 8;                  // i
 constexpr int kGvColPairIter = 4;              // j
-constexpr int kGvTR = 16 * kGvRowIter;         // 256 rows
+constexpr int kGvTR = 16 * kGvRowIter;         // 128 rows
 constexpr int kGvTC = 2 * 32 * kGvColPairIter; // 256 cols
 constexpr int kGvTileElems = kGvTR * kGvTC;
 

@@ -4,6 +4,7 @@
 // captured once into a hipGraph and replayed.  Step control lives in
 // device memory (Ctl), so the host only re-launches the graph in chunks and
 // drains the per-step log between chunks.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -83,6 +84,7 @@ struct mpci_system {
   std::vector<int> nrb, ncb;   // per pair: row tiles, column tiles
   long long S = 0;             // sum_L s_L s_{L+1} (dipole elements)
   int device = 0;
+  int gv_grid = 512;  // persistent GEMV workgroups: 2 per compute unit
   // row-major copy of every D_L for the batched (GEMM) form, built on the
   // first propagator with P >= 2 from the host blocks kept until then
   DevBuf<double> Dr;
@@ -323,6 +325,10 @@ extern "C" int mpci_system_create(int nb, const int *s, char gauge,
   sys->off.assign(nb + 1, 0);
   for (int L = 0; L < nb; ++L) sys->off[L + 1] = sys->off[L] + s[L];
   (void)hipGetDevice(&sys->device);
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                            sys->device) == hipSuccess && ncu > 0)
+    sys->gv_grid = 2 * ncu;
   int rc = build_system(sys, E, D);
   if (rc != MPCI_OK) {
     delete sys;
@@ -333,6 +339,20 @@ extern "C" int mpci_system_create(int nb, const int *s, char gauge,
   for (int L = 0; L + 1 < nb; ++L)
     sys->hostD[L].assign(D[L], D[L] + (size_t)s[L] * s[L + 1]);
   *out = sys;
+  return MPCI_OK;
+}
+
+extern "C" int mpci_system_set_gemv_grid(mpci_system *sys, int workgroups) {
+  if (!sys || workgroups < 0)
+    return fail(MPCI_ERR_ARG, "mpci_system_set_gemv_grid: bad argument");
+  if (workgroups == 0) {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                              sys->device) != hipSuccess || ncu <= 0)
+      ncu = 256;
+    workgroups = 2 * ncu;
+  }
+  sys->gv_grid = workgroups;
   return MPCI_OK;
 }
 
@@ -434,15 +454,18 @@ static int build_gemm_jobs(mpci_prop *pr, int wm, int wn, int ks) {
   return MPCI_OK;
 }
 
-static int launch_gemm(mpci_prop *pr, const double2 *xin) {
+// e0 / e1 (may be null): HIP events attached to the kernel's own dispatch,
+// so their interval is the kernel's execution alone.
+static int launch_gemm(mpci_prop *pr, const double2 *xin,
+                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   mpci_system *sys = pr->sys;
   const dim3 grid((unsigned)pr->gjobs.size());
   const double *X = reinterpret_cast<const double *>(xin);
   double *part = reinterpret_cast<double *>(pr->part.p);
-#define MPCI_GEMM(WM, WN)                                                    \
-  hipLaunchKernelGGL((k_gemm_coupling<WM, WN>), grid, dim3(64 * WM * WN), 0, \
-                     pr->stream, pr->dgjobs.p, sys->Dr.p, sys->dr_off.p, X,  \
-                     part, pr->desc, pr->ctl.p)
+#define MPCI_GEMM(WM, WN)                                                  \
+  hipExtLaunchKernelGGL((k_gemm_coupling<WM, WN>), grid, dim3(64 * WM * WN), \
+                        0, pr->stream, e0, e1, 0, pr->dgjobs.p, sys->Dr.p,  \
+                        sys->dr_off.p, X, part, pr->desc, pr->ctl.p)
   if (pr->gm_wm == 2 && pr->gm_wn == 2)
     MPCI_GEMM(2, 2);
   else if (pr->gm_wm == 2 && pr->gm_wn == 4)
@@ -630,17 +653,23 @@ extern "C" int mpci_prop_set_pulse(mpci_prop *pr, int p,
 static int enqueue_tile(mpci_prop *pr, const double2 *xin, int s) {
   mpci_system *sys = pr->sys;
   if (sys->jobs.empty()) return MPCI_OK;
-  const bool ev = pr->instrument;
-  if (ev) HIPCHK(hipEventRecord(pr->ev[pr->ev_base + 2 * s], pr->stream));
+  // instrumented: the event pair rides on the kernel's dispatch packet
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (pr->instrument) {
+    e0 = pr->ev[pr->ev_base + 2 * s];
+    e1 = pr->ev[pr->ev_base + 2 * s + 1];
+  }
   if (pr->P == 1) {
-    hipLaunchKernelGGL(k_gemv_tile, dim3((unsigned)sys->jobs.size()),
-                       dim3(kGvThreads), 0, pr->stream, sys->djobs.p,
-                       sys->Dg.p, xin, pr->part.p, pr->desc, pr->ctl.p);
+    const int njobs = (int)sys->jobs.size();
+    hipExtLaunchKernelGGL(k_gemv_tile,
+                          dim3((unsigned)std::min(njobs, sys->gv_grid)),
+                          dim3(kGvThreads), 0, pr->stream, e0, e1, 0,
+                          sys->djobs.p, njobs, sys->Dg.p, xin, pr->part.p,
+                          pr->desc, pr->ctl.p);
     HIPCHK(hipGetLastError());
   } else {
-    TRY(launch_gemm(pr, xin));
+    TRY(launch_gemm(pr, xin, e0, e1));
   }
-  if (ev) HIPCHK(hipEventRecord(pr->ev[pr->ev_base + 2 * s + 1], pr->stream));
   return MPCI_OK;
 }
 

@@ -155,134 +155,188 @@ __device__ __forceinline__ double add_xor32(double v) {
 }
 
 // ---------------------------------------------------------------------
-// Fused single-pass coupling over one 256 x 256 tile of D_L.
+// Fused single-pass coupling, persistent over TR x TC units (tiles) of the
+// dipole blocks.  Workgroup w streams units w, w + G, w + 2G, ... (G =
+// gridDim.x); the workgroups that run together therefore read one
+// contiguous front of the interleaved store.  While a unit's epilogue runs
+// (column sums, partial stores, row sums), the first row group of the next
+// unit is already in flight and its x segments are staged into the other
+// half of the double-buffered LDS, so the HBM stream has no idle phase
+// between units.
 // ---------------------------------------------------------------------
+namespace {
+struct UnitGeo {
+  int L, rb, cb, sL, sL1, row0, col0, nrows, nI;
+};
+
+__device__ __forceinline__ UnitGeo unit_geo(const TileJob *jobs, int u,
+                                            const SysDesc &d) {
+  const TileJob jb = jobs[u];
+  UnitGeo q;
+  q.L = jb.L;
+  q.rb = jb.rb;
+  q.cb = jb.cb;
+  q.sL = d.off[q.L + 1] - d.off[q.L];
+  q.sL1 = d.off[q.L + 2] - d.off[q.L + 1];
+  q.row0 = jb.rb * kGvTR;
+  q.col0 = jb.cb * kGvTC;
+  q.nrows = min(kGvTR, q.sL - q.row0);
+  q.nI = (q.nrows + 15) >> 4;
+  return q;
+}
+}  // namespace
+
 __global__ __launch_bounds__(kGvThreads) void k_gemv_tile(
-    const TileJob *__restrict__ jobs, const double *__restrict__ Dg,
+    const TileJob *__restrict__ jobs, int njobs, const double *__restrict__ Dg,
     const double2 *__restrict__ X, double2 *__restrict__ part, SysDesc d,
     const Ctl *__restrict__ ctl) {
-  const TileJob jb = jobs[blockIdx.x];
   if (ctl[0].done) return;
-  const int L = jb.L;
-  const int sL = d.off[L + 1] - d.off[L];
-  const int sL1 = d.off[L + 2] - d.off[L + 1];
-  const double2 *x0 = X + d.off[L];
-  const double2 *x1 = X + d.off[L + 1];
+  const int G = gridDim.x;
+  int u = blockIdx.x;
+  if (u >= njobs) return;
   const int t = threadIdx.x;
   const int tr = t & 15, g = t >> 4, lane = t & 63, w = t >> 6;
-  const int row0 = jb.rb * kGvTR, col0 = jb.cb * kGvTC;
-  const int nrows = min(kGvTR, sL - row0);
 
-  __shared__ double2 xs0[kGvTR];
-  __shared__ double2 rowbuf[kGvThreads / 64][kGvTR];
+  __shared__ double2 xs0[2][kGvTR];   // x_L over the unit's rows
+  __shared__ double2 xs1[2][kGvTC];   // x_{L+1} over the unit's columns
+  __shared__ double2 rowbuf[2][kGvThreads / 64][kGvTR];
 
-  bool cv[kGvColPairIter];
-#pragma unroll
-  for (int j = 0; j < kGvColPairIter; ++j) cv[j] = col0 + 2 * (g + 32 * j) < sL1;
-
-  // Software-pipelined stream over the tile: the loads of row group i + 1
-  // are in flight while row group i is consumed.  Every load is issued
-  // unconditionally: a lane whose row or column pair lies outside the block
-  // re-reads its own j = 0 element of the group, which lies inside the
-  // zero-padded tile store and is fetched by the j = 0 load anyway.  Such a
-  // value only meets exact zeros (xs0 and xc are zero outside the block) or
-  // feeds partials that are never stored, so it needs no masking.
-  // D is streamed once, so its loads are non-temporal.
+  // Every load is issued unconditionally within a unit: a lane whose row or
+  // column pair lies outside the block re-reads its own j = 0 element of
+  // the group, which lies inside the zero-padded tile store and is fetched
+  // by the j = 0 load anyway.  Such a value only meets exact zeros (xs0 and
+  // xs1 are zero outside the block) or feeds partials that are never
+  // stored, so it needs no masking.  D is streamed once: non-temporal loads.
   typedef double dv2 __attribute__((ext_vector_type(2)));
-  const dv2 *Dt = reinterpret_cast<const dv2 *>(Dg) +
-                  (long long)blockIdx.x * kGvColPairIter * kGvThreads + t;
-  const long long gstride = (long long)gridDim.x * kGvColPairIter * kGvThreads;
-  const int nI = (nrows + 15) >> 4;  // uniform over the workgroup
-  auto load_group = [&](int i, double2 (&v)[kGvColPairIter]) {
+  const dv2 *D2 = reinterpret_cast<const dv2 *>(Dg) + t;
+  // group i of unit uu, whose block has nrows rows in this unit and whose
+  // columns run from col0 to sL1 (all uniform over the workgroup)
+  auto load_group = [&](int uu, int i, int nrows, int col0, int sL1,
+                        dv2 (&v)[kGvColPairIter]) {
     const bool rv = (tr + 16 * i) < nrows;
-    const dv2 *base = Dt + i * gstride;
+    const dv2 *base =
+        D2 + ((long long)i * njobs + uu) * (kGvColPairIter * kGvThreads);
 #pragma unroll
     for (int j = 0; j < kGvColPairIter; ++j) {
-      int o = (rv && cv[j]) ? j * kGvThreads : 0;
-      // opaque to the optimiser: keeps four independent vector loads
-      // instead of a branch that reuses the j = 0 load
+      const bool cv = col0 + 2 * (g + 32 * j) < sL1;
+      int o = (rv && cv) ? j * kGvThreads : 0;
+      // opaque to the optimiser: keeps independent vector loads instead
+      // of a branch that reuses the j = 0 load
       asm volatile("" : "+v"(o));
-      const dv2 d = __builtin_nontemporal_load(base + o);
-      v[j] = make_double2(d.x, d.y);
+      v[j] = __builtin_nontemporal_load(base + o);
     }
   };
-  // the first row group's loads go out before the x loads, so the tile's
-  // stream starts one memory latency earlier
-  double2 cur[kGvColPairIter], nxt[kGvColPairIter];
-  load_group(0, cur);
+  auto stage_x = [&](const UnitGeo &q, int b) {
+    const double2 *x0 = X + d.off[q.L];
+    const double2 *x1 = X + d.off[q.L + 1];
+    for (int r = t; r < kGvTR; r += kGvThreads)
+      xs0[b][r] = (r < q.nrows) ? x0[q.row0 + r] : make_double2(0.0, 0.0);
+    for (int c = t; c < kGvTC; c += kGvThreads)
+      xs1[b][c] =
+          (q.col0 + c < q.sL1) ? x1[q.col0 + c] : make_double2(0.0, 0.0);
+  };
 
-  double2 xc[kGvColPairIter][2];
-  double2 acc[kGvColPairIter][2];
-#pragma unroll
-  for (int j = 0; j < kGvColPairIter; ++j) {
-    const int c = col0 + 2 * (g + 32 * j);
-    xc[j][0] = (c < sL1) ? x1[c] : make_double2(0.0, 0.0);
-    xc[j][1] = (c + 1 < sL1) ? x1[c + 1] : make_double2(0.0, 0.0);
-    acc[j][0] = make_double2(0.0, 0.0);
-    acc[j][1] = make_double2(0.0, 0.0);
-  }
-  for (int r = t; r < kGvTR; r += kGvThreads)
-    xs0[r] = (r < nrows) ? x0[row0 + r] : make_double2(0.0, 0.0);
+  UnitGeo q = unit_geo(jobs, u, d);
+  dv2 cur[kGvColPairIter], nxt[kGvColPairIter];
+  load_group(u, 0, q.nrows, q.col0, q.sL1, cur);
+  stage_x(q, 0);
   __syncthreads();
-
-  for (int i = 0; i < nI; ++i) {
-    // unconditional prefetch (the last one re-reads group nI-1 from cache)
-    // keeps the wait counters exact across iterations
-    load_group(min(i + 1, nI - 1), nxt);
-    const int r = tr + 16 * i;
-    const double2 xr = xs0[r];
-    double pre = 0.0, pim = 0.0;
+  int b = 0;
+  for (;;) {
+    const int un = u + G;
+    const bool more = un < njobs;  // uniform over the workgroup
+    const UnitGeo qn = more ? unit_geo(jobs, un, d) : q;
+    // Stage the next unit's x into the other LDS half now: its loads are
+    // waited for together with this unit's first row group, which the
+    // first iteration needs anyway.  That half was last read before the
+    // previous unit's barrier, which every wave has passed.
+    if (more) stage_x(qn, b ^ 1);
+    double2 acc[kGvColPairIter][2];
 #pragma unroll
     for (int j = 0; j < kGvColPairIter; ++j) {
-      const double2 dv = cur[j];
-      pre = fma(dv.x, xc[j][0].x, pre);
-      pim = fma(dv.x, xc[j][0].y, pim);
-      pre = fma(dv.y, xc[j][1].x, pre);
-      pim = fma(dv.y, xc[j][1].y, pim);
-      acc[j][0].x = fma(dv.x, xr.x, acc[j][0].x);
-      acc[j][0].y = fma(dv.x, xr.y, acc[j][0].y);
-      acc[j][1].x = fma(dv.y, xr.x, acc[j][1].x);
-      acc[j][1].y = fma(dv.y, xr.y, acc[j][1].y);
+      acc[j][0] = make_double2(0.0, 0.0);
+      acc[j][1] = make_double2(0.0, 0.0);
     }
-    // sum the row partial over the 4 column groups held by this wave
-    pre = add_xor32(add_xor16(pre));
-    pim = add_xor32(add_xor16(pim));
-    if (lane < 16) rowbuf[w][r] = make_double2(pre, pim);
+    for (int i = 0; i < q.nI; ++i) {
+      // The next group of this unit, or the next unit's first group, or
+      // (after the workgroup's last unit) this group again, a cache hit.
+      // One load site with uniform operands keeps the loads in flight.
+      const bool last = i + 1 == q.nI;
+      const bool nu = last && more;
+      load_group(nu ? un : u, last ? (nu ? 0 : i) : i + 1,
+                 nu ? qn.nrows : q.nrows, nu ? qn.col0 : q.col0,
+                 nu ? qn.sL1 : q.sL1, nxt);
+      const int r = tr + 16 * i;
+      const double2 xr = xs0[b][r];
+      double pre = 0.0, pim = 0.0;
 #pragma unroll
-    for (int j = 0; j < kGvColPairIter; ++j) cur[j] = nxt[j];
-  }
+      for (int j = 0; j < kGvColPairIter; ++j) {
+        const dv2 dv = cur[j];
+        const double2 xa = xs1[b][2 * (g + 32 * j)];
+        const double2 xb = xs1[b][2 * (g + 32 * j) + 1];
+        pre = fma(dv.x, xa.x, pre);
+        pim = fma(dv.x, xa.y, pim);
+        pre = fma(dv.y, xb.x, pre);
+        pim = fma(dv.y, xb.y, pim);
+        acc[j][0].x = fma(dv.x, xr.x, acc[j][0].x);
+        acc[j][0].y = fma(dv.x, xr.y, acc[j][0].y);
+        acc[j][1].x = fma(dv.y, xr.x, acc[j][1].x);
+        acc[j][1].y = fma(dv.y, xr.y, acc[j][1].y);
+      }
+      // sum the row partial over the 4 column groups held by this wave
+      pre = add_xor32(add_xor16(pre));
+      pim = add_xor32(add_xor16(pim));
+      if (lane < 16) rowbuf[b][w][r] = make_double2(pre, pim);
+      if (i + 1 < q.nI) {
+#pragma unroll
+        for (int j = 0; j < kGvColPairIter; ++j) cur[j] = nxt[j];
+      }
+    }
 
-  // column partials: sum over the 16 row lanes (lane bits 0..3)
-#pragma unroll
-  for (int j = 0; j < kGvColPairIter; ++j) {
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      acc[j][e].x = sum16(acc[j][e].x);
-      acc[j][e].y = sum16(acc[j][e].y);
-    }
-  }
-  if (tr == 0) {
-    double2 *pc = part + d.col_part_off[L] + (long long)jb.rb * sL1;
+    // ---- epilogue of unit u (the next unit's first group is in flight)
+    // column partials: sum over the 16 row lanes (lane bits 0..3)
 #pragma unroll
     for (int j = 0; j < kGvColPairIter; ++j) {
-      const int c = col0 + 2 * (g + 32 * j);
-      if (c < sL1) pc[c] = acc[j][0];
-      if (c + 1 < sL1) pc[c + 1] = acc[j][1];
-    }
-  }
-  __syncthreads();
-  // row partials: sum over the 8 waves in a fixed order
-  {
-    const int r = t >> 1, comp = t & 1;
-    if (r < nrows) {
-      double v = 0.0;
 #pragma unroll
-      for (int ww = 0; ww < kGvThreads / 64; ++ww)
-        v += comp ? rowbuf[ww][r].y : rowbuf[ww][r].x;
-      double *pr = reinterpret_cast<double *>(part + d.row_part_off[L] +
-                                              (long long)jb.cb * sL + row0 + r);
-      pr[comp] = v;
+      for (int e = 0; e < 2; ++e) {
+        acc[j][e].x = sum16(acc[j][e].x);
+        acc[j][e].y = sum16(acc[j][e].y);
+      }
     }
+    if (tr == 0) {
+      double2 *pc = part + d.col_part_off[q.L] + (long long)q.rb * q.sL1;
+#pragma unroll
+      for (int j = 0; j < kGvColPairIter; ++j) {
+        const int c = q.col0 + 2 * (g + 32 * j);
+        if (c < q.sL1) pc[c] = acc[j][0];
+        if (c + 1 < q.sL1) pc[c + 1] = acc[j][1];
+      }
+    }
+    __syncthreads();
+    // row partials: sum over the 8 waves in a fixed order
+    {
+      const int r = t >> 1, comp = t & 1;
+      if (r < q.nrows) {
+        double v = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < kGvThreads / 64; ++ww)
+          v += comp ? rowbuf[b][ww][r].y : rowbuf[b][ww][r].x;
+        double *pr = reinterpret_cast<double *>(
+            part + d.row_part_off[q.L] + (long long)q.cb * q.sL + q.row0 + r);
+        pr[comp] = v;
+      }
+    }
+    if (!more) break;
+    // Only now take the next unit's first group: the opaque use pins the
+    // copy (and the wait for those loads) after the epilogue's stores.
+#pragma unroll
+    for (int j = 0; j < kGvColPairIter; ++j) {
+      asm volatile("" : "+v"(nxt[j]) : : "memory");
+      cur[j] = nxt[j];
+    }
+    u = un;
+    q = qn;
+    b ^= 1;
   }
 }
 

@@ -48,7 +48,7 @@ struct AdjArgs {
 __global__ void k_relayout_gemv(const double *D, int sL, int sL1, int ncb,
                                 long long nelem, int job0, int njobs,
                                 double *out);
-__global__ void k_gemv_tile(const TileJob *jobs, const double *Dg,
+__global__ void k_gemv_tile(const TileJob *jobs, int njobs, const double *Dg,
                             const double2 *X, double2 *part, SysDesc d,
                             const Ctl *ctl);
 template <int WM, int WN>

@@ -69,6 +69,7 @@ _SIGS = [
     ("mpci_system_create", C.c_int, [C.c_int, _ip, C.c_char, C.POINTER(_dp),
                                      C.POINTER(_dp), C.POINTER(_vp)]),
     ("mpci_system_destroy", C.c_int, [_vp]),
+    ("mpci_system_set_gemv_grid", C.c_int, [_vp, C.c_int]),
     ("mpci_system_info", C.c_int, [_vp, _ip, C.POINTER(C.c_longlong)]),
     ("mpci_prop_create", C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
     ("mpci_prop_destroy", C.c_int, [_vp]),

@@ -91,6 +91,11 @@ class System:
         return int(sum(int(self.s[L]) * int(self.s[L + 1])
                        for L in range(len(self.s) - 1)))
 
+    def set_gemv_grid(self, workgroups: int):
+        """Workgroups of the persistent single-pulse coupling kernel (0:
+        the default, 2 per compute unit)."""
+        check(_lib.tdse_lib().mpci_system_set_gemv_grid(self.h, workgroups))
+
     def close(self):
         if self.h:
             _lib.tdse_lib().mpci_system_destroy(self.h)

@@ -140,6 +140,22 @@ def test_graph_and_eager_agree(mpci, oracle):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("grid", [1, 3, 7, 0])
+def test_persistent_gemv_grid(mpci, oracle, grid):
+    """The persistent coupling kernel gives the same RHS whatever the number
+    of workgroups (each then streams many tiles, ragged last tiles
+    included)."""
+    g, o = build(mpci, oracle, [300, 420, 270, 150], "v", seed=12)
+    g.set_gemv_grid(grid)
+    prop = mpci.Propagator(g)
+    rng = np.random.default_rng(grid)
+    x = rng.standard_normal(2 * g.N)
+    got = prop.rhs(0.37, x)
+    ref = o.rhs(0.37, x)
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    g.set_gemv_grid(0)
+
+
 def test_instrumented_runs_agree(mpci, oracle):
     """The instrumented mode (eager launches with an event pair around every
     coupling kernel) gives the graph result bit for bit and counts 13
