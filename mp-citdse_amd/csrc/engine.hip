@@ -913,6 +913,9 @@ static int plan_chunk(const mpci_prop *pr, const std::vector<Ctl> &c) {
     } else {
       e = x.nsteps - x.step;
     }
+    // never launch attempts past a pulse's attempt limit (they would only
+    // return at once)
+    e = std::min(e, x.max_attempts - x.attempts);
     est = std::max(est, e);
   }
   if (!any) return 0;
