@@ -197,10 +197,12 @@ int mpci_prop_kernel_time(mpci_prop *prop, double *total_ms,
                           long long *launches, long long *bytes_per_launch,
                           long long *flops_per_launch);
 /* Batched form (P >= 2): the MFMA tile (waves along M and N, each wave a
- * 32 x 32 block) and K split the autotuner chose at mpci_prop_create, and
- * its measured time per coupling evaluation in ms. */
+ * 32 x 32 block), K split, kernel (0: single LDS buffer, 1: double-buffered
+ * with 16-byte loads) and K chunk the autotuner chose at mpci_prop_create,
+ * and its measured time per coupling evaluation in ms.  Any pointer may be
+ * NULL. */
 int mpci_prop_gemm_config(mpci_prop *prop, int *wm, int *wn, int *ks,
-                          double *ms);
+                          double *ms, int *kind, int *tk);
 
 #ifdef __cplusplus
 }

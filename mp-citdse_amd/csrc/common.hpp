@@ -52,7 +52,7 @@ struct GemmJob {
   int n0;       // first real column
   int k0, k1;   // K range [k0, k1)
   int q;        // partial slot (K-split index)
-  int pad;
+  int ld;       // row stride of D_L in the row-major copy (even)
 };
 
 // Per-propagator description of the blocks and the partial-sum buffers.

@@ -89,11 +89,12 @@ struct mpci_system {
   // first propagator with P >= 2 from the host blocks kept until then
   DevBuf<double> Dr;
   DevBuf<long long> dr_off;
+  std::vector<int> dr_ld;  // even row stride of each block in Dr
   std::vector<std::vector<double>> hostD;
   // autotuned GEMM tile per batch size P: every propagator of this system
   // with the same P uses the same tiles, hence the same summation order
   struct Tuned {
-    int wm, wn, ks;
+    int kind, wm, wn, tk, ks;
     double ms;
   };
   std::map<int, Tuned> tuned;
@@ -124,7 +125,7 @@ struct mpci_prop {
   hipGraph_t graph = nullptr;
   int graph_track = -1;
   // batched form: chosen MFMA tile (waves along M, N) and K splits
-  int gm_wm = 2, gm_wn = 2, gm_ks = 1;
+  int gm_kind = 0, gm_wm = 2, gm_wn = 2, gm_tk = 32, gm_ks = 1;
   double gemm_ms = 0.0;  // autotuned time of one coupling evaluation
   std::vector<GemmJob> gjobs;
   DevBuf<GemmJob> dgjobs;
@@ -399,29 +400,37 @@ static int init_desc(mpci_prop *pr, int ks) {
   return MPCI_OK;
 }
 
+// Row-major copy of every D_L with an even row stride ld_L (the padding
+// column, when there is one, holds zeros), so the batched kernels can load
+// 16-byte pairs from any even column.
 static int ensure_rowmajor(mpci_system *sys) {
   if (sys->Dr.p || sys->nb < 2) return MPCI_OK;
   std::vector<long long> off(sys->nb - 1);
+  sys->dr_ld.assign(sys->nb - 1, 0);
   long long tot = 0;
   for (int L = 0; L + 1 < sys->nb; ++L) {
+    sys->dr_ld[L] = (sys->s[L + 1] + 1) & ~1;
     off[L] = tot;
-    tot += (long long)sys->s[L] * sys->s[L + 1];
+    tot += (long long)sys->s[L] * sys->dr_ld[L];
   }
   TRY(sys->Dr.alloc(tot));
   TRY(sys->dr_off.alloc(off.size()));
+  HIPCHK(hipMemset(sys->Dr.p, 0, sizeof(double) * tot));
   HIPCHK(hipMemcpy(sys->dr_off.p, off.data(), sizeof(long long) * off.size(),
                    hipMemcpyHostToDevice));
   for (int L = 0; L + 1 < sys->nb; ++L)
-    HIPCHK(hipMemcpy(sys->Dr.p + off[L], sys->hostD[L].data(),
-                     sizeof(double) * sys->hostD[L].size(),
-                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy2D(sys->Dr.p + off[L], sizeof(double) * sys->dr_ld[L],
+                       sys->hostD[L].data(), sizeof(double) * sys->s[L + 1],
+                       sizeof(double) * sys->s[L + 1], sys->s[L],
+                       hipMemcpyHostToDevice));
   HIPCHK(hipDeviceSynchronize());
   return MPCI_OK;
 }
 
 // GEMM jobs: for every pair and direction, output panels of 32*wm rows x
 // 32*wn real columns, each over one of `ks` equal K ranges.
-static int build_gemm_jobs(mpci_prop *pr, int wm, int wn, int ks) {
+static int build_gemm_jobs(mpci_prop *pr, int kind, int wm, int wn, int tk,
+                           int ks) {
   mpci_system *sys = pr->sys;
   const int TM = 32 * wm, TN = 32 * wn, P2 = 2 * pr->P;
   pr->gjobs.clear();
@@ -441,6 +450,7 @@ static int build_gemm_jobs(mpci_prop *pr, int wm, int wn, int ks) {
             j.k0 = std::min(K, q * kchunk);
             j.k1 = std::min(K, (q + 1) * kchunk);
             j.q = q;
+            j.ld = sys->dr_ld[L];
             pr->gjobs.push_back(j);
           }
     }
@@ -448,8 +458,10 @@ static int build_gemm_jobs(mpci_prop *pr, int wm, int wn, int ks) {
   if (!pr->gjobs.empty())
     TRY(h2d(pr, pr->dgjobs.p, pr->gjobs.data(),
             sizeof(GemmJob) * pr->gjobs.size()));
+  pr->gm_kind = kind;
   pr->gm_wm = wm;
   pr->gm_wn = wn;
+  pr->gm_tk = tk;
   pr->gm_ks = ks;
   return MPCI_OK;
 }
@@ -466,15 +478,41 @@ static int launch_gemm(mpci_prop *pr, const double2 *xin,
   hipExtLaunchKernelGGL((k_gemm_coupling<WM, WN>), grid, dim3(64 * WM * WN), \
                         0, pr->stream, e0, e1, 0, pr->dgjobs.p, sys->Dr.p,  \
                         sys->dr_off.p, X, part, pr->desc, pr->ctl.p)
-  if (pr->gm_wm == 2 && pr->gm_wn == 2)
-    MPCI_GEMM(2, 2);
-  else if (pr->gm_wm == 2 && pr->gm_wn == 4)
-    MPCI_GEMM(2, 4);
-  else if (pr->gm_wm == 4 && pr->gm_wn == 2)
-    MPCI_GEMM(4, 2);
-  else
-    MPCI_GEMM(4, 4);
+#define MPCI_GEMM_DB(WM, WN, TK)                                              \
+  hipExtLaunchKernelGGL((k_gemm_db<WM, WN, TK>), grid, dim3(64 * WM * WN),    \
+                        0, pr->stream, e0, e1, 0, pr->dgjobs.p, sys->Dr.p,   \
+                        sys->dr_off.p, X, part, pr->desc, pr->ctl.p)
+  const int wm = pr->gm_wm, wn = pr->gm_wn, tk = pr->gm_tk;
+  if (pr->gm_kind == 0) {
+    if (wm == 2 && wn == 2)
+      MPCI_GEMM(2, 2);
+    else if (wm == 2 && wn == 4)
+      MPCI_GEMM(2, 4);
+    else if (wm == 4 && wn == 2)
+      MPCI_GEMM(4, 2);
+    else
+      MPCI_GEMM(4, 4);
+  } else if (tk == 32) {
+    if (wm == 2 && wn == 4)
+      MPCI_GEMM_DB(2, 4, 32);
+    else
+      MPCI_GEMM_DB(4, 2, 32);
+  } else {
+    if (wm == 2 && wn == 1)
+      MPCI_GEMM_DB(2, 1, 16);
+    else if (wm == 4 && wn == 1)
+      MPCI_GEMM_DB(4, 1, 16);
+    else if (wm == 2 && wn == 2)
+      MPCI_GEMM_DB(2, 2, 16);
+    else if (wm == 4 && wn == 2)
+      MPCI_GEMM_DB(4, 2, 16);
+    else if (wm == 2 && wn == 4)
+      MPCI_GEMM_DB(2, 4, 16);
+    else
+      MPCI_GEMM_DB(4, 4, 16);
+  }
 #undef MPCI_GEMM
+#undef MPCI_GEMM_DB
   HIPCHK(hipGetLastError());
   return MPCI_OK;
 }
@@ -487,16 +525,24 @@ static int autotune_gemm(mpci_prop *pr) {
   if (it != pr->sys->tuned.end()) {
     const auto &t = it->second;
     TRY(init_desc(pr, t.ks));
-    TRY(build_gemm_jobs(pr, t.wm, t.wn, t.ks));
+    TRY(build_gemm_jobs(pr, t.kind, t.wm, t.wn, t.tk, t.ks));
     pr->gemm_ms = t.ms;
     return MPCI_OK;
   }
+  // kind 0: k_gemm_coupling (single LDS buffer, K chunks of 32);
+  // kind 1: k_gemm_db (double-buffered, 16-byte loads, K chunks of tk)
   struct Cand {
-    int wm, wn, ks;
+    int kind, wm, wn, tk, ks;
   };
-  const Cand cands[] = {{2, 2, 1}, {2, 2, 2}, {2, 4, 1}, {2, 4, 2},
-                        {4, 2, 1}, {4, 2, 2}, {4, 4, 1}, {4, 4, 2},
-                        {4, 4, 4}, {2, 4, 4}};
+  const Cand cands[] = {
+      {0, 2, 2, 32, 1}, {0, 2, 2, 32, 2}, {0, 2, 4, 32, 1}, {0, 2, 4, 32, 2},
+      {0, 4, 2, 32, 1}, {0, 4, 2, 32, 2}, {0, 4, 4, 32, 1}, {0, 4, 4, 32, 2},
+      {0, 4, 4, 32, 4}, {0, 2, 4, 32, 4},
+      {1, 2, 1, 16, 1}, {1, 2, 1, 16, 2}, {1, 4, 1, 16, 1}, {1, 4, 1, 16, 2},
+      {1, 2, 2, 16, 1}, {1, 2, 2, 16, 2}, {1, 4, 2, 16, 1}, {1, 4, 2, 16, 2},
+      {1, 2, 4, 16, 1}, {1, 2, 4, 16, 2}, {1, 2, 4, 16, 4}, {1, 4, 4, 16, 1},
+      {1, 4, 4, 16, 2}, {1, 2, 4, 32, 1}, {1, 2, 4, 32, 2}, {1, 4, 2, 32, 1},
+      {1, 4, 2, 32, 2}, {1, 4, 2, 16, 4}};
   std::vector<Ctl> c(pr->P);
   std::memset(c.data(), 0, sizeof(Ctl) * pr->P);  // all pulses live
   TRY(h2d(pr, pr->ctl.p, c.data(), sizeof(Ctl) * pr->P));
@@ -507,7 +553,7 @@ static int autotune_gemm(mpci_prop *pr) {
   Cand bc = cands[0];
   for (const Cand &cd : cands) {
     TRY(init_desc(pr, cd.ks));
-    TRY(build_gemm_jobs(pr, cd.wm, cd.wn, cd.ks));
+    TRY(build_gemm_jobs(pr, cd.kind, cd.wm, cd.wn, cd.tk, cd.ks));
     TRY(launch_gemm(pr, pr->X.p));  // warm
     HIPCHK(hipEventRecord(e0, pr->stream));
     for (int r = 0; r < 3; ++r) TRY(launch_gemm(pr, pr->X.p));
@@ -525,9 +571,9 @@ static int autotune_gemm(mpci_prop *pr) {
   for (auto &x : c) x.done = 1;
   TRY(h2d(pr, pr->ctl.p, c.data(), sizeof(Ctl) * pr->P));
   TRY(init_desc(pr, bc.ks));
-  TRY(build_gemm_jobs(pr, bc.wm, bc.wn, bc.ks));
+  TRY(build_gemm_jobs(pr, bc.kind, bc.wm, bc.wn, bc.tk, bc.ks));
   pr->gemm_ms = best / 3.0;
-  pr->sys->tuned[pr->P] = {bc.wm, bc.wn, bc.ks, pr->gemm_ms};
+  pr->sys->tuned[pr->P] = {bc.kind, bc.wm, bc.wn, bc.tk, bc.ks, pr->gemm_ms};
   return MPCI_OK;
 }
 
@@ -1422,11 +1468,13 @@ extern "C" int mpci_prop_kernel_time(mpci_prop *pr, double *total_ms,
 }
 
 extern "C" int mpci_prop_gemm_config(mpci_prop *pr, int *wm, int *wn, int *ks,
-                                     double *ms) {
+                                     double *ms, int *kind, int *tk) {
   if (!pr) return fail(MPCI_ERR_ARG, "NULL argument");
   if (wm) *wm = pr->gm_wm;
   if (wn) *wn = pr->gm_wn;
   if (ks) *ks = pr->gm_ks;
   if (ms) *ms = pr->gemm_ms;
+  if (kind) *kind = pr->gm_kind;
+  if (tk) *tk = pr->gm_tk;
   return MPCI_OK;
 }

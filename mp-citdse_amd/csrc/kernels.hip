@@ -410,8 +410,8 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_coupling(
       const int gm = jb.m0 + m, gk = k0 + k;
       double v = 0.0;
       if (idx < TM * TK && gm < Mdim && gk < jb.k1)
-        v = jb.dir == 0 ? D[(long long)gm * sL1 + gk]
-                        : D[(long long)gk * sL1 + gm];
+        v = jb.dir == 0 ? D[(long long)gm * jb.ld + gk]
+                        : D[(long long)gk * jb.ld + gm];
       ra[u] = v;
     }
 #pragma unroll
@@ -489,6 +489,189 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_coupling(
         if (m < Mdim && n < P2) out[(long long)m * P2 + n] = acc[bi][bj][r];
       }
 }
+
+// ---------------------------------------------------------------------
+// Batched coupling on FP64 MFMA, double-buffered.  Same panels, operand
+// layouts and partial slots as k_gemm_coupling; K advances in chunks of TK
+// through two LDS buffers: while the MFMAs consume chunk c, chunk c + 1 is
+// already in the other buffer and chunk c + 2 is in flight from HBM as
+// 16-byte loads (the row-major dipole copy has an even row stride, and its
+// padding and the ragged edges read as zeros), so one barrier per chunk
+// separates the phases.
+// ---------------------------------------------------------------------
+template <int WM, int WN, int TK>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_db(
+    const GemmJob *__restrict__ jobs, const double *__restrict__ Dr,
+    const long long *__restrict__ dr_off, const double *__restrict__ X,
+    double *__restrict__ part, SysDesc d, const Ctl *__restrict__ ctl) {
+  constexpr int TM = 32 * WM, TN = 32 * WN;
+  constexpr int NTH = 64 * WM * WN;
+  constexpr int LDA = TM + 2, LDB = TN + 2;  // padded, even LDS rows
+  constexpr int NPA = TM * TK / 2, NPB = TK * TN / 2;  // 16-byte pairs
+  constexpr int NA = (NPA + NTH - 1) / NTH;
+  constexpr int NB = (NPB + NTH - 1) / NTH;
+  __shared__ double As[2][TK * LDA];
+  __shared__ double Bs[2][TK * LDB];
+  __shared__ int live;
+
+  const GemmJob jb = jobs[blockIdx.x];
+  const int t = threadIdx.x;
+  const int P2 = 2 * d.P;
+  // skip the panel when every pulse it covers has finished
+  if (t == 0) live = 0;
+  __syncthreads();
+  {
+    const int p0 = jb.n0 >> 1, p1 = min(d.P, (jb.n0 + TN + 1) >> 1);
+    for (int p = p0 + t; p < p1; p += NTH)
+      if (!ctl[p].done) live = 1;
+  }
+  __syncthreads();
+  if (!live) return;
+
+  const int L = jb.L;
+  const int sL = d.off[L + 1] - d.off[L];
+  const int sL1 = d.off[L + 2] - d.off[L + 1];
+  const int Mdim = jb.dir == 0 ? sL : sL1;
+  const double *D = Dr + dr_off[L];
+  const double *Xk =
+      X + (long long)(jb.dir == 0 ? d.off[L + 1] : d.off[L]) * P2;
+  const int ld = jb.ld;
+
+  typedef double dv2 __attribute__((ext_vector_type(2)));
+  dv2 ra[NA], rb[NB];
+  // chunk loads: A pairs run along k (dir 0: rows of D) or along m (dir 1:
+  // rows of D are the k index), B pairs along n; a pair is loaded whole when
+  // its first element is inside the panel, its second element then being
+  // inside the block or in the zero padding of the row
+  auto load_chunk = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int pi = t + u * NTH;
+      dv2 v = dv2{0.0, 0.0};
+      if (pi < NPA) {
+        if (jb.dir == 0) {
+          const int m = pi / (TK / 2), k = 2 * (pi % (TK / 2));
+          const int gm = jb.m0 + m, gk = k0 + k;
+          if (gm < Mdim && gk < jb.k1)
+            v = *reinterpret_cast<const dv2 *>(D + (long long)gm * ld + gk);
+        } else {
+          const int k = pi / (TM / 2), m = 2 * (pi % (TM / 2));
+          const int gm = jb.m0 + m, gk = k0 + k;
+          if (gm < Mdim && gk < jb.k1)
+            v = *reinterpret_cast<const dv2 *>(D + (long long)gk * ld + gm);
+        }
+      }
+      ra[u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int pi = t + u * NTH;
+      dv2 v = dv2{0.0, 0.0};
+      if (pi < NPB) {
+        const int k = pi / (TN / 2), n = 2 * (pi % (TN / 2));
+        const int gk = k0 + k, gn = jb.n0 + n;
+        if (gk < jb.k1 && gn < P2)
+          v = *reinterpret_cast<const dv2 *>(Xk + (long long)gk * P2 + gn);
+      }
+      rb[u] = v;
+    }
+  };
+  auto store_chunk = [&](int b) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int pi = t + u * NTH;
+      if (pi < NPA) {
+        if (jb.dir == 0) {
+          const int m = pi / (TK / 2), k = 2 * (pi % (TK / 2));
+          As[b][k * LDA + m] = ra[u].x;
+          As[b][(k + 1) * LDA + m] = ra[u].y;
+        } else {
+          const int k = pi / (TM / 2), m = 2 * (pi % (TM / 2));
+          *reinterpret_cast<dv2 *>(&As[b][k * LDA + m]) = ra[u];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int pi = t + u * NTH;
+      if (pi < NPB) {
+        const int k = pi / (TN / 2), n = 2 * (pi % (TN / 2));
+        *reinterpret_cast<dv2 *>(&Bs[b][k * LDB + n]) = rb[u];
+      }
+    }
+  };
+
+  const int lane = t & 63, w = t >> 6;
+  const int wm = w % WM, wn = w / WM;
+  const int li = lane & 15, lk = lane >> 4;
+  dmf4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = dmf4{0.0, 0.0, 0.0, 0.0};
+
+  const int nch = (jb.k1 - jb.k0 + TK - 1) / TK;
+  load_chunk(jb.k0);
+  store_chunk(0);
+  if (nch > 1) load_chunk(jb.k0 + TK);
+  __syncthreads();
+  for (int c = 0; c < nch; ++c) {
+    const int b = c & 1;
+    // chunk c + 1 into the other buffer (read last in chunk c - 1, before
+    // the previous barrier), then chunk c + 2 into flight
+    if (c + 1 < nch) {
+      store_chunk(b ^ 1);
+      if (c + 2 < nch) load_chunk(jb.k0 + (c + 2) * TK);
+    }
+#pragma unroll
+    for (int ks = 0; ks < TK / 4; ++ks) {
+      const int kr = ks * 4 + lk;
+      double a[2], bb[2];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+        a[bi] = As[b][kr * LDA + wm * 32 + bi * 16 + li];
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+        bb[bj] = Bs[b][kr * LDB + wn * 32 + bj * 16 + li];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              a[bi], bb[bj], acc[bi][bj], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+  // write the panel into partial slot q of this pair and direction
+  const long long base =
+      jb.dir == 0 ? d.row_part_off[L] : d.col_part_off[L];  // double2 units
+  double *out = part + 2 * base + (long long)jb.q * Mdim * P2;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = jb.m0 + wm * 32 + bi * 16 + 4 * r + lk;
+        const int n = jb.n0 + wn * 32 + bj * 16 + li;
+        if (m < Mdim && n < P2) out[(long long)m * P2 + n] = acc[bi][bj][r];
+      }
+}
+
+#define MPCI_GEMM_DB(WM, WN, TK)                                              \
+  template __global__ void k_gemm_db<WM, WN, TK>(                            \
+      const GemmJob *, const double *, const long long *, const double *,    \
+      double *, SysDesc, const Ctl *);
+MPCI_GEMM_DB(2, 1, 16)
+MPCI_GEMM_DB(4, 1, 16)
+MPCI_GEMM_DB(2, 2, 16)
+MPCI_GEMM_DB(4, 2, 16)
+MPCI_GEMM_DB(2, 4, 16)
+MPCI_GEMM_DB(4, 4, 16)
+MPCI_GEMM_DB(2, 4, 32)
+MPCI_GEMM_DB(4, 2, 32)
+#undef MPCI_GEMM_DB
 
 template __global__ void k_gemm_coupling<2, 2>(const GemmJob *, const double *,
                                                const long long *,

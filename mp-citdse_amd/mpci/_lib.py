@@ -98,7 +98,7 @@ _SIGS = [
     ("mpci_prop_kernel_time", C.c_int, [_vp, _dp, C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_longlong)]),
-    ("mpci_prop_gemm_config", C.c_int, [_vp, _ip, _ip, _ip, _dp]),
+    ("mpci_prop_gemm_config", C.c_int, [_vp, _ip, _ip, _ip, _dp, _ip, _ip]),
 ]
 
 # Every symbol the header declares; tests check the library exports them.

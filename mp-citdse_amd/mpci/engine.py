@@ -238,12 +238,15 @@ class Propagator:
         return ms.value, n.value, b.value, f.value
 
     def gemm_config(self):
-        wm, wn, ks = C.c_int(), C.c_int(), C.c_int()
+        wm, wn, ks, kind, tk = (C.c_int(), C.c_int(), C.c_int(), C.c_int(),
+                                C.c_int())
         ms = C.c_double()
         check(_lib.tdse_lib().mpci_prop_gemm_config(
-            self.h, C.byref(wm), C.byref(wn), C.byref(ks), C.byref(ms)))
-        return {"wm": wm.value, "wn": wn.value, "ksplit": ks.value,
-                "ms": ms.value}
+            self.h, C.byref(wm), C.byref(wn), C.byref(ks), C.byref(ms),
+            C.byref(kind), C.byref(tk)))
+        return {"kernel": ("k_gemm_coupling", "k_gemm_db")[kind.value],
+                "wm": wm.value, "wn": wn.value, "tk": tk.value,
+                "ksplit": ks.value, "ms": ms.value}
 
 
 def multi_adaptive(props, t0, t1, h0, atol, rtol, max_attempts=10_000_000,
