@@ -106,7 +106,8 @@ struct mpci_prop {
   hipStream_t stream = nullptr;
   SysDesc desc{};
   long long NP = 0;
-  int nblk = 0;           // k_stage workgroups
+  int nblk = 0;           // workgroups of the per-row kernels (256 rows / P)
+  int nblk_stage = 0;     // k_stage workgroups (64 rows for one pulse)
   DevBuf<double2> X, Xs, K, Y, part;
   DevBuf<double> red;
   DevBuf<Ctl> ctl;
@@ -591,12 +592,14 @@ extern "C" int mpci_prop_create(mpci_system *sys, int P, mpci_prop **out) {
   pr->NP = (long long)sys->N * P;
   const int rpb = 256 / P;
   pr->nblk = (sys->N + rpb - 1) / rpb;
+  const int rps = P == 1 ? 64 : rpb;  // k_stage<4> for one pulse
+  pr->nblk_stage = (sys->N + rps - 1) / rps;
   TRY(init_desc(pr.get(), 1));
   TRY(pr->X.alloc(pr->NP));
   TRY(pr->Xs.alloc(pr->NP));
   TRY(pr->Y.alloc(pr->NP));
   TRY(pr->K.alloc(pr->NP * kStages));
-  TRY(pr->red.alloc((size_t)pr->nblk * P * 2));
+  TRY(pr->red.alloc((size_t)std::max(pr->nblk, pr->nblk_stage) * P * 2));
   TRY(pr->ctl.alloc(P));
   TRY(pr->pulses.alloc(P));
   TRY(pr->log.alloc((size_t)pr->logcap * P));
@@ -719,6 +722,17 @@ static int enqueue_tile(mpci_prop *pr, const double2 *xin, int s) {
   return MPCI_OK;
 }
 
+static int launch_stage(mpci_prop *pr, const StageArgs &a) {
+  if (pr->P == 1)
+    hipLaunchKernelGGL(k_stage<4>, dim3(pr->nblk_stage), dim3(256), 0,
+                       pr->stream, a, pr->desc);
+  else
+    hipLaunchKernelGGL(k_stage<1>, dim3(pr->nblk_stage), dim3(256), 0,
+                       pr->stream, a, pr->desc);
+  HIPCHK(hipGetLastError());
+  return MPCI_OK;
+}
+
 static StageArgs stage_args(mpci_prop *pr, int s) {
   StageArgs a{};
   a.X = pr->X.p;
@@ -741,12 +755,11 @@ static StageArgs stage_args(mpci_prop *pr, int s) {
 static int enqueue_attempt(mpci_prop *pr) {
   for (int s = 0; s < kStages; ++s) {
     TRY(enqueue_tile(pr, s == 0 ? pr->X.p : pr->Xs.p, s));
-    hipLaunchKernelGGL(k_stage, dim3(pr->nblk), dim3(256), 0, pr->stream,
-                       stage_args(pr, s), pr->desc);
+    TRY(launch_stage(pr, stage_args(pr, s)));
     HIPCHK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_finalize, dim3(pr->P), dim3(256), 0, pr->stream,
-                     pr->ctl.p, pr->red.p, pr->nblk, pr->P, pr->hs.p);
+                     pr->ctl.p, pr->red.p, pr->nblk_stage, pr->P, pr->hs.p);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_apply, dim3(pr->nblk), dim3(256), 0, pr->stream,
                      pr->X.p, pr->Y.p, pr->ctl.p, pr->log.p, pr->logcap,
@@ -887,8 +900,7 @@ extern "C" int mpci_prop_rhs(mpci_prop *pr, const double *F, const double *x,
   TRY(enqueue_tile(pr, pr->X.p, 0));
   StageArgs a = stage_args(pr, 0);
   a.red = nullptr;
-  hipLaunchKernelGGL(k_stage, dim3(pr->nblk), dim3(256), 0, pr->stream, a,
-                     pr->desc);
+  TRY(launch_stage(pr, a));
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(pr->stream));
   std::vector<double2> k(pr->NP);

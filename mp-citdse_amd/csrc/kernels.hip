@@ -717,15 +717,53 @@ __device__ __forceinline__ double2 sum_partials(const double2 *base, int n,
   return acc;
 }
 
+// Partials q = sub, sub + SUB, ... of both sums (column partials of D_{L-1}^T
+// x_{L-1}, row partials of D_L x_{L+1}), up to 8 of each in flight at once.
+template <int SUB>
+__device__ __forceinline__ void sum_both(const double2 *cb, int nc,
+                                         const double2 *rb, int nr,
+                                         long long stride, long long at,
+                                         int sub, double2 &cs, double2 &rs) {
+  cs = make_double2(0.0, 0.0);
+  rs = make_double2(0.0, 0.0);
+  const int n = nc > nr ? nc : nr;
+  for (int q0 = sub; q0 < n; q0 += 8 * SUB) {
+    double2 vc[8], vr[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int q = q0 + u * SUB;
+      vc[u] = q < nc ? cb[(long long)q * stride + at] : make_double2(0.0, 0.0);
+      vr[u] = q < nr ? rb[(long long)q * stride + at] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      cs.x += vc[u].x;
+      cs.y += vc[u].y;
+      rs.x += vr[u].x;
+      rs.y += vr[u].y;
+    }
+  }
+}
+
+// Stage kernel.  SUB lanes share one (row, pulse): they split the partial
+// sums (every SUB-th partial each) and combine them with a DPP quad
+// reduction, lane 0 of the group doing the rest.  SUB = 4 for one pulse
+// (dozens of partials per row), 1 for the batched form (K-split partials).
+template <int SUB>
 __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
+  static_assert(SUB == 1 || SUB == 4, "lanes per row: 1 or a DPP quad");
   const int P = d.P;
-  const int rpb = 256 / P;
   const int t = threadIdx.x;
-  const int lr = t / P, p = t - lr * P;
+  const int sub = t & (SUB - 1);
+  const int tt = t / SUB;
+  const int rpb = (256 / SUB) / P;
+  const int lr = tt / P, p = tt - lr * P;
   const int row = blockIdx.x * rpb + lr;
   const bool active_lane = (lr < rpb) && (row < d.N);
   const int s = a.s;
   double errv = 0.0, nrm2 = 0.0;
+  // every lane of a SUB group has the same row and pulse, so the group is
+  // active or inactive as a whole (the DPP reads stay inside it)
   if (active_lane) {
     const Ctl &c = a.ctl[p];
     if (!c.done) {
@@ -733,82 +771,92 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
       const int i = row - d.off[L];
       const int sL = d.off[L + 1] - d.off[L];
       const long long idx = (long long)row * P + p;
-      double2 cs = make_double2(0.0, 0.0), rs = make_double2(0.0, 0.0);
-      // Partials are read 8 at a time so their loads are in flight together.
-      if (L > 0)
-        cs = sum_partials(a.part + d.col_part_off[L - 1], d.ncol_part[L - 1],
-                          (long long)sL * P, (long long)i * P + p);
-      if (L < d.nb - 1)
-        rs = sum_partials(a.part + d.row_part_off[L], d.nrow_part[L],
-                          (long long)sL * P, (long long)i * P + p);
-      const double2 xs = (s == 0) ? a.X[idx] : a.Xs[idx];
-      const double e = a.E[row];
-      double F;
-      if (c.fmode == kFieldTable)
-        F = a.Ftab[(long long)p * c.hstride + c.step];
-      else if (c.fmode == kFieldPulse)
-        F = field_at(a.pulses[p], a.env, c.t + cC[s] * c.hc);
-      else
-        F = c.Fc;
-      double2 k;
-      if (d.gauge == 0) {  // velocity: -iE x + F (D^T x_{L-1} - D x_{L+1})
-        k.x = e * xs.y + F * (cs.x - rs.x);
-        k.y = -e * xs.x + F * (cs.y - rs.y);
-      } else {             // length: -i (E x + F (D^T x_{L-1} + D x_{L+1}))
-        const double cx = cs.x + rs.x, cy = cs.y + rs.y;
-        k.x = e * xs.y + F * cy;
-        k.y = -e * xs.x - F * cx;
+      double2 cs, rs;
+      sum_both<SUB>(a.part + (L > 0 ? d.col_part_off[L - 1] : 0),
+                    L > 0 ? d.ncol_part[L - 1] : 0,
+                    a.part + (L < d.nb - 1 ? d.row_part_off[L] : 0),
+                    L < d.nb - 1 ? d.nrow_part[L] : 0, (long long)sL * P,
+                    (long long)i * P + p, sub, cs, rs);
+      if (SUB == 4) {
+        cs.x += dpp_d<0xB1>(cs.x);
+        cs.y += dpp_d<0xB1>(cs.y);
+        rs.x += dpp_d<0xB1>(rs.x);
+        rs.y += dpp_d<0xB1>(rs.y);
+        cs.x += dpp_d<0x4E>(cs.x);
+        cs.y += dpp_d<0x4E>(cs.y);
+        rs.x += dpp_d<0x4E>(rs.x);
+        rs.y += dpp_d<0x4E>(rs.y);
       }
-      const long long NP = a.NP;
-      a.K[s * NP + idx] = k;
-      if (c.mode != kModeRhs) {
-        const double h = c.hc;
-        const double2 x = a.X[idx];
-        if (s < kStages - 1) {
-          double vx = 1.0 * x.x, vy = 1.0 * x.y;
-          double2 kj[kStages - 1];
-#pragma unroll
-          for (int j = 0; j < kStages - 1; ++j)
-            kj[j] = (j < s) ? a.K[j * NP + idx] : make_double2(0.0, 0.0);
-#pragma unroll
-          for (int j = 0; j < kStages - 1; ++j) {
-            if (j < s) {
-              const double co = h * cA[s + 1][j];
-              vx += co * kj[j].x;
-              vy += co * kj[j].y;
-            }
-          }
-          const double co = h * cA[s + 1][s];
-          vx += co * k.x;
-          vy += co * k.y;
-          a.Xs[idx] = make_double2(vx, vy);
-          if (a.xs_store)
-            a.xs_store[((long long)c.step * (kStages - 1) + s) * NP + idx] =
-                make_double2(vx, vy);
-        } else {
-          double yx = 1.0 * x.x, yy = 1.0 * x.y, ex = 0.0, ey = 0.0;
-          double2 kj[kStages];
-#pragma unroll
-          for (int j = 0; j < kStages - 1; ++j) kj[j] = a.K[j * NP + idx];
-          kj[kStages - 1] = k;
-          const double2 k1 = kj[0];
-#pragma unroll
-          for (int j = 0; j < kStages; ++j) {
-            const double cb = h * cB[j], cd = h * cDB[j];
-            yx += cb * kj[j].x;
-            yy += cb * kj[j].y;
-            ex += cd * kj[j].x;
-            ey += cd * kj[j].y;
-          }
-          a.Y[idx] = make_double2(yx, yy);
-          const double rx =
-              fabs(ex) / (c.atol + c.rtol * (fabs(x.x) + h * fabs(k1.x)));
-          const double ry =
-              fabs(ey) / (c.atol + c.rtol * (fabs(x.y) + h * fabs(k1.y)));
-          errv = rx > ry ? rx : ry;
-          nrm2 = yx * yx + yy * yy;
+      if (sub == 0) {
+        const double2 xs = (s == 0) ? a.X[idx] : a.Xs[idx];
+        const double e = a.E[row];
+        double F;
+        if (c.fmode == kFieldTable)
+          F = a.Ftab[(long long)p * c.hstride + c.step];
+        else if (c.fmode == kFieldPulse)
+          F = field_at(a.pulses[p], a.env, c.t + cC[s] * c.hc);
+        else
+          F = c.Fc;
+        double2 k;
+        if (d.gauge == 0) {  // velocity: -iE x + F (D^T x_{L-1} - D x_{L+1})
+          k.x = e * xs.y + F * (cs.x - rs.x);
+          k.y = -e * xs.x + F * (cs.y - rs.y);
+        } else {             // length: -i (E x + F (D^T x_{L-1} + D x_{L+1}))
+          const double cx = cs.x + rs.x, cy = cs.y + rs.y;
+          k.x = e * xs.y + F * cy;
+          k.y = -e * xs.x - F * cx;
         }
-      }
+        const long long NP = a.NP;
+        a.K[s * NP + idx] = k;
+        if (c.mode != kModeRhs) {
+          const double h = c.hc;
+          const double2 x = a.X[idx];
+          if (s < kStages - 1) {
+            double vx = 1.0 * x.x, vy = 1.0 * x.y;
+            double2 kj[kStages - 1];
+#pragma unroll
+            for (int j = 0; j < kStages - 1; ++j)
+              kj[j] = (j < s) ? a.K[j * NP + idx] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int j = 0; j < kStages - 1; ++j) {
+              if (j < s) {
+                const double co = h * cA[s + 1][j];
+                vx += co * kj[j].x;
+                vy += co * kj[j].y;
+              }
+            }
+            const double co = h * cA[s + 1][s];
+            vx += co * k.x;
+            vy += co * k.y;
+            a.Xs[idx] = make_double2(vx, vy);
+            if (a.xs_store)
+              a.xs_store[((long long)c.step * (kStages - 1) + s) * NP + idx] =
+                  make_double2(vx, vy);
+          } else {
+            double yx = 1.0 * x.x, yy = 1.0 * x.y, ex = 0.0, ey = 0.0;
+            double2 kj[kStages];
+#pragma unroll
+            for (int j = 0; j < kStages - 1; ++j) kj[j] = a.K[j * NP + idx];
+            kj[kStages - 1] = k;
+            const double2 k1 = kj[0];
+#pragma unroll
+            for (int j = 0; j < kStages; ++j) {
+              const double cb = h * cB[j], cd = h * cDB[j];
+              yx += cb * kj[j].x;
+              yy += cb * kj[j].y;
+              ex += cd * kj[j].x;
+              ey += cd * kj[j].y;
+            }
+            a.Y[idx] = make_double2(yx, yy);
+            const double rx =
+                fabs(ex) / (c.atol + c.rtol * (fabs(x.x) + h * fabs(k1.x)));
+            const double ry =
+                fabs(ey) / (c.atol + c.rtol * (fabs(x.y) + h * fabs(k1.y)));
+            errv = rx > ry ? rx : ry;
+            nrm2 = yx * yx + yy * yy;
+          }
+        }
+      }  // sub == 0
     }
   }
   if (s == kStages - 1 && a.red != nullptr) {
@@ -819,15 +867,17 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
     if (t < P) {
       double em = 0.0, ns = 0.0;
       for (int r = 0; r < rpb; ++r) {
-        const double v = se[r * P + t];
+        const double v = se[(r * P + t) * SUB];
         em = v > em ? v : em;
-        ns += sn[r * P + t];
+        ns += sn[(r * P + t) * SUB];
       }
       a.red[((long long)blockIdx.x * P + t) * 2 + 0] = em;
       a.red[((long long)blockIdx.x * P + t) * 2 + 1] = ns;
     }
   }
 }
+template __global__ void k_stage<1>(StageArgs, SysDesc);
+template __global__ void k_stage<4>(StageArgs, SysDesc);
 
 // ---------------------------------------------------------------------
 // Per-pulse reduction and step control; one workgroup per pulse.

@@ -127,10 +127,12 @@ def test_batched_adjoint_uneven_pulses(mpci, oracle):
         # the batch couples on MFMA, the single pulse on the GEMV kernel:
         # their sums differ in the last bits, the accepted step sizes then
         # differ by ~1e-7 relative (the embedded error cancels ~10 digits),
-        # the final states by up to ~1e-10 (the adaptive-mode bar) and the
-        # gradient of each discrete map by ~1e-10 relative
+        # the final states by up to ~1e-10 (the adaptive-mode bar), and the
+        # gradients of the two discrete maps (different schedules) by up to
+        # that relative step difference; each is held to its own finite
+        # differences at 1e-6 below
         assert abs(J1[0] - J[p]) < 1e-10
-        assert rel(grad[p], g1[0]) < 1e-8
+        assert rel(grad[p], g1[0]) < 1e-6
         fd = gpu_fd(mpci, g, envs[p], w, Ts[p], "v", hlog[p, :nst[p]], W, x0)
         assert rel(grad[p], fd) < 1e-6
 
