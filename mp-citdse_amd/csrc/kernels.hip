@@ -257,21 +257,15 @@ __global__ __launch_bounds__(kGvThreads) void k_gemv_tile(
       acc[j][0] = make_double2(0.0, 0.0);
       acc[j][1] = make_double2(0.0, 0.0);
     }
-    for (int i = 0; i < q.nI; ++i) {
-      // The next group of this unit, or the next unit's first group, or
-      // (after the workgroup's last unit) this group again, a cache hit.
-      // One load site with uniform operands keeps the loads in flight.
-      const bool last = i + 1 == q.nI;
-      const bool nu = last && more;
-      load_group(nu ? un : u, last ? (nu ? 0 : i) : i + 1,
-                 nu ? qn.nrows : q.nrows, nu ? qn.col0 : q.col0,
-                 nu ? qn.sL1 : q.sL1, nxt);
+    // one row group: products with the unit's x segments, the row sums of
+    // this wave into rowbuf, the column sums into acc
+    auto consume = [&](int i, const dv2 (&cv)[kGvColPairIter]) {
       const int r = tr + 16 * i;
       const double2 xr = xs0[b][r];
       double pre = 0.0, pim = 0.0;
 #pragma unroll
       for (int j = 0; j < kGvColPairIter; ++j) {
-        const dv2 dv = cur[j];
+        const dv2 dv = cv[j];
         const double2 xa = xs1[b][2 * (g + 32 * j)];
         const double2 xb = xs1[b][2 * (g + 32 * j) + 1];
         pre = fma(dv.x, xa.x, pre);
@@ -287,11 +281,21 @@ __global__ __launch_bounds__(kGvThreads) void k_gemv_tile(
       pre = add_xor32(add_xor16(pre));
       pim = add_xor32(add_xor16(pim));
       if (lane < 16) rowbuf[b][w][r] = make_double2(pre, pim);
-      if (i + 1 < q.nI) {
+    };
+    // groups 0 .. nI-2: the next group of this unit is in flight meanwhile
+    for (int i = 0; i + 1 < q.nI; ++i) {
+      load_group(u, i + 1, q.nrows, q.col0, q.sL1, nxt);
+      consume(i, cur);
 #pragma unroll
-        for (int j = 0; j < kGvColPairIter; ++j) cur[j] = nxt[j];
-      }
+      for (int j = 0; j < kGvColPairIter; ++j) cur[j] = nxt[j];
     }
+    // last group: the next unit's first group goes into flight (after the
+    // workgroup's last unit, this group again: a cache hit) and stays in
+    // flight through the epilogue
+    load_group(more ? un : u, more ? 0 : q.nI - 1,
+               more ? qn.nrows : q.nrows, more ? qn.col0 : q.col0,
+               more ? qn.sL1 : q.sL1, nxt);
+    consume(q.nI - 1, cur);
 
     // ---- epilogue of unit u (the next unit's first group is in flight)
     // column partials: sum over the 16 row lanes (lane bits 0..3)
