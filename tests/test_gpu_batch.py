@@ -115,3 +115,66 @@ def test_batched_equals_single_pulse_runs(mpci):
         st1, _ = one.prop_adaptive(0.0, t1[p], 0.05, 1e-10, 1e-10)
         assert st1[0].accepted == stb[p].accepted
         assert np.abs(one.get_state() - batch.get_state(p)).max() < 1e-11
+
+
+@pytest.mark.parametrize("s", [[1, 1], [5], [3, 2, 4], [1, 700, 2], [45, 31]])
+def test_batched_rhs_degenerate_shapes(mpci, oracle, s):
+    """Single-state blocks, one block (no coupling), odd sizes (the padded
+    row stride of the row-major copy) in the batched form."""
+    P = 5
+    g, o = build(mpci, oracle, s, "l", seed=sum(s) + 1)
+    prop = mpci.Propagator(g, P=P)
+    rng = np.random.default_rng(len(s))
+    x = rng.standard_normal((P, 2 * g.N))
+    F = rng.uniform(-1, 1, P)
+    got = prop.rhs(F, x.reshape(-1)).reshape(P, -1)
+    for p in range(P):
+        ref = o.rhs(F[p], x[p])
+        assert np.abs(got[p] - ref).max() <= 1e-13 * max(1, np.abs(ref).max())
+
+
+def test_batched_rhs_max_pulses(mpci, oracle):
+    """P = 256, the largest batch the interface accepts; 0 and 257 are
+    refused with MPCI_ERR_ARG."""
+    g, o = build(mpci, oracle, [40, 33, 20], "v", seed=77)
+    P = 256
+    prop = mpci.Propagator(g, P=P)
+    rng = np.random.default_rng(256)
+    x = rng.standard_normal((P, 2 * g.N))
+    F = rng.uniform(-1, 1, P)
+    got = prop.rhs(F, x.reshape(-1)).reshape(P, -1)
+    for p in (0, 1, 127, 255):
+        ref = o.rhs(F[p], x[p])
+        assert np.abs(got[p] - ref).max() <= 1e-13 * max(1, np.abs(ref).max())
+    for bad in (0, 257):
+        with pytest.raises(mpci.MpciError) as e:
+            mpci.Propagator(g, P=bad)
+        assert e.value.code == -1
+
+
+def test_empty_runs_leave_state(mpci, oracle):
+    """Zero fixed steps and an empty adaptive interval change nothing; a
+    batch where some pulses have an empty interval still runs the others."""
+    from mpci import synth
+    g, o = build(mpci, oracle, [30, 20, 25], "v", seed=8)
+    P = 3
+    prop = mpci.Propagator(g, P=P)
+    rng = np.random.default_rng(9)
+    xs = [rng.standard_normal(2 * g.N) for _ in range(P)]
+    xs = [x / np.linalg.norm(x) for x in xs]
+    for p in range(P):
+        prop.set_state(xs[p], p)
+        prop.set_pulse(mpci.Pulse.sine(0.03 ** 2, 0.7, 2), p)
+    assert prop.prop_fixed(0.0, 0.1, np.zeros((P, 0))).shape == (P, 0)
+    st, _ = prop.prop_adaptive(1.0, 1.0, 0.05, 1e-10, 1e-10)
+    assert all(s.accepted == 0 and s.attempts == 0 for s in st)
+    for p in range(P):
+        assert np.array_equal(prop.get_state(p), xs[p])
+    t1 = np.array([1.0, 0.0, 2.0])  # pulse 1: empty interval
+    st, _ = prop.prop_adaptive(0.0, t1, 0.05, 1e-10, 1e-10)
+    assert st[1].accepted == 0 and np.array_equal(prop.get_state(1), xs[1])
+    for p in (0, 2):
+        po = oracle.Pulse.sine(0.03 ** 2, 0.7, 2)
+        xr, sr, _ = o.prop_adaptive(po, 0.0, t1[p], 0.05, 1e-10, 1e-10, xs[p])
+        assert st[p].accepted == sr.accepted
+        assert np.abs(prop.get_state(p) - xr).max() < 1e-10
