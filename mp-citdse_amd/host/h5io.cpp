@@ -72,21 +72,26 @@ std::vector<double> read_block(const std::string &file,
   return out;
 }
 
-void write_block(const std::string &file, const std::string &dataset,
-                 const double *data, int rank, const hsize_t *dims) {
-  quiet();
-  H5Id f(must(H5Fcreate(file.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT),
-              "cannot create HDF5 file " + file),
-         H5Fclose);
+void write_into(hid_t f, const std::string &dataset, const double *data,
+                int rank, const hsize_t *dims) {
   H5Id space(must(H5Screate_simple(rank, dims, nullptr), "dataspace"),
              H5Sclose);
-  H5Id d(must(H5Dcreate2(f.id, dataset.c_str(), H5T_IEEE_F64LE, space.id,
+  H5Id d(must(H5Dcreate2(f, dataset.c_str(), H5T_IEEE_F64LE, space.id,
                          H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
               "cannot create dataset " + dataset),
          H5Dclose);
   must_ok(H5Dwrite(d.id, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT,
                    data),
           "write of " + dataset);
+}
+
+void write_block(const std::string &file, const std::string &dataset,
+                 const double *data, int rank, const hsize_t *dims) {
+  quiet();
+  H5Id f(must(H5Fcreate(file.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT),
+              "cannot create HDF5 file " + file),
+         H5Fclose);
+  write_into(f.id, dataset, data, rank, dims);
 }
 
 }  // namespace
@@ -132,6 +137,27 @@ void h5_write_2d(const std::string &file, const std::string &dataset,
                  const double *data, long rows, long cols) {
   hsize_t d[2] = {(hsize_t)rows, (hsize_t)cols};
   write_block(file, dataset, data, 2, d);
+}
+
+std::vector<double> h5_read_all(const std::string &file,
+                                const std::string &dataset,
+                                std::vector<long> *dims) {
+  const std::vector<long> d = h5_dims(file, dataset);
+  if (dims) *dims = d;
+  if (d.size() == 1) return h5_read_prefix_1d(file, dataset, d[0]);
+  if (d.size() == 2) return h5_read_block_2d(file, dataset, d[0], d[1]);
+  throw H5Error("dataset " + dataset + " in " + file + " is not 1-D or 2-D");
+}
+
+void h5_write_all(const std::string &file, const std::vector<H5Out> &sets) {
+  quiet();
+  H5Id f(must(H5Fcreate(file.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT),
+              "cannot create HDF5 file " + file),
+         H5Fclose);
+  for (const H5Out &o : sets) {
+    std::vector<hsize_t> d(o.dims.begin(), o.dims.end());
+    write_into(f.id, o.name, o.data, (int)d.size(), d.data());
+  }
 }
 
 }  // namespace mpci

@@ -29,8 +29,21 @@ std::vector<double> h5_read_block_2d(const std::string &file,
 // Extent of a dataset (1 or 2 dims).
 std::vector<long> h5_dims(const std::string &file, const std::string &dataset);
 
+// Read a whole 1-D or 2-D dataset (row-major), with its extent.
+std::vector<double> h5_read_all(const std::string &file,
+                                const std::string &dataset,
+                                std::vector<long> *dims = nullptr);
+
 void h5_write_1d(const std::string &file, const std::string &dataset,
                  const double *data, long n);
+// Several datasets into one new file (w2e.cpp:67-71 writes En_CI and
+// CI_vecs side by side).
+struct H5Out {
+  std::string name;
+  const double *data;
+  std::vector<long> dims;
+};
+void h5_write_all(const std::string &file, const std::vector<H5Out> &sets);
 void h5_write_2d(const std::string &file, const std::string &dataset,
                  const double *data, long rows, long cols);
 

@@ -6,13 +6,20 @@
 //         (w2e.cpp:67-71, 115-120), dat/<pot>0.h5:/En (the 1e threshold
 //         pes reads, pes.cpp:168-176) and placeholder dat/cfg-<L>.inp files
 //         (tdse copies them, tdse_main.cpp:122-126).
-// The values come from host/synth.c, so `mpci.synth.system_arrays` with
-// the same sizes and seed reproduces them bit for bit.
+//   -w:   instead of the CI files, the product-basis inputs the w2e stage
+//         reads (w2e.cpp:25-42, 93-102): dat/<pot>2_<L>En.h5:/e_2e,
+//         dat/<pot>V12_<L>.h5:/V_12 (packed upper triangle) and
+//         dat/<pot>2_<L><L+1><g>.h5:/d_if, plus the threshold and cfg files.
+// The values come from host/synth.c, so `mpci.synth.system_arrays` (and
+// `mpci.synth.w2e_arrays` for -w) with the same sizes and seed reproduces
+// them bit for bit.
 //
 // usage: tdse_gen -e 1|2 -p <pot> -s n0,n1,... -g v|l [-d <dir>]
-//                 [-r <seed>] [-t <threshold>]
+//                 [-r <seed>] [-t <threshold>] [-w]
 #include <unistd.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <filesystem>
@@ -25,16 +32,50 @@
 #include "h5io.hpp"
 #include "synth.h"
 
+namespace {
+
+// Seeded product-basis inputs of the w2e stage: e_2e ~ U[-2.9, 5] sorted,
+// V_12 entries ~ N(0, (0.2 / sqrt(n))^2) as the packed upper triangle, d_if
+// entries ~ N(0, 1 / n_{L+1}).  Streams 5000 + L, 6000 + L and 7000 + L
+// of the generator in host/synth.c.
+int mpci_synth_w2e(uint64_t seed, int nb, const int *s,
+                   std::vector<std::vector<double>> *ens,
+                   std::vector<std::vector<double>> *v12,
+                   std::vector<std::vector<double>> *dip) {
+  ens->assign(nb, {});
+  v12->assign(nb, {});
+  dip->assign(nb > 1 ? nb - 1 : 0, {});
+  for (int L = 0; L < nb; ++L) {
+    const long long n = s[L];
+    (*ens)[L].resize(n);
+    mpci_synth_uniform(seed, 5000 + L, n, -2.9, 5.0, (*ens)[L].data());
+    std::sort((*ens)[L].begin(), (*ens)[L].end());
+    (*v12)[L].resize(n * (n + 1) / 2);
+    mpci_synth_normal(seed, 6000 + L, n * (n + 1) / 2, 0.2 / std::sqrt(n),
+                      (*v12)[L].data());
+  }
+  for (int L = 0; L + 1 < nb; ++L) {
+    (*dip)[L].resize((size_t)s[L] * s[L + 1]);
+    mpci_synth_normal(seed, 7000 + L, (long long)s[L] * s[L + 1],
+                      1.0 / std::sqrt((double)s[L + 1]), (*dip)[L].data());
+  }
+  return 0;
+}
+
+}  // namespace
+
 int main(int argc, char *argv[]) {
   int e_num = 0;
   std::string pot, dir = "dat", sizes;
   char gauge = 'v';
   uint64_t seed = 20261202;
   double threshold = -2.0;
+  bool w2e_inputs = false;
   for (;;) {
-    int c = getopt(argc, argv, "e:p:s:g:d:r:t:");
+    int c = getopt(argc, argv, "e:p:s:g:d:r:t:w");
     if (c == -1) break;
     switch (c) {
+    case 'w': w2e_inputs = true; break;
     case 'e': e_num = std::atoi(optarg); break;
     case 'p': pot = optarg; break;
     case 's': sizes = optarg; break;
@@ -63,6 +104,39 @@ int main(int argc, char *argv[]) {
       return 2;
     }
   const int nb = (int)s.size();
+  if (w2e_inputs) {
+    try {
+      std::filesystem::create_directories(dir);
+      const std::string base = dir + "/" + pot;
+      std::vector<std::vector<double>> ens, v12, dip;
+      if (mpci_synth_w2e(seed, nb, s.data(), &ens, &v12, &dip) != 0) {
+        std::cerr << "tdse_gen: synthesis failed\n";
+        return 1;
+      }
+      for (int L = 0; L < nb; ++L) {
+        const std::string l = std::to_string(L);
+        mpci::h5_write_1d(base + "2_" + l + "En.h5", "e_2e", ens[L].data(),
+                          s[L]);
+        mpci::h5_write_1d(base + "V12_" + l + ".h5", "V_12", v12[L].data(),
+                          (long)v12[L].size());
+      }
+      for (int L = 0; L + 1 < nb; ++L) {
+        const std::string ll =
+            std::to_string(L) + std::to_string(L + 1) + gauge;
+        mpci::h5_write_2d(base + "2_" + ll + ".h5", "d_if", dip[L].data(),
+                          s[L], s[L + 1]);
+      }
+      mpci::h5_write_1d(base + "0.h5", "En", &threshold, 1);
+      for (int L = 0; L < nb; ++L) {
+        std::ofstream f(dir + "/cfg-" + std::to_string(L) + ".inp");
+        f << "# synthetic configuration list placeholder for L=" << L << "\n";
+      }
+    } catch (const std::exception &ex) {
+      std::cerr << "tdse_gen: " << ex.what() << "\n";
+      return 1;
+    }
+    return 0;
+  }
   std::vector<std::vector<double>> E(nb), D(nb > 1 ? nb - 1 : 0);
   std::vector<double *> Ep(nb), Dp(nb > 1 ? nb - 1 : 1, nullptr);
   for (int L = 0; L < nb; ++L) {

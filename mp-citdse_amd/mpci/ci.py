@@ -1,0 +1,68 @@
+"""Python side of the CI basis producer (the w2e stage) over the C ABI in
+include/mpci_ci.h; the arithmetic runs in lib/libmpci_ci.so on the GPU."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _lib
+
+CI_LIB = os.path.join(_lib.LIB_DIR, "libmpci_ci.so")
+EXPORTED = ["mpci_ci_last_error", "mpci_ci_basis", "mpci_ci_dipoles"]
+_ci = None
+
+
+def ci_lib():
+    global _ci
+    if _ci is None:
+        if not os.path.exists(CI_LIB):
+            raise RuntimeError(f"{CI_LIB} is missing: build it with "
+                               "`python -c 'import __graft_entry__ as g; "
+                               "g.build()'`")
+        lib = C.CDLL(CI_LIB)
+        lib.mpci_ci_last_error.restype = C.c_char_p
+        lib.mpci_ci_last_error.argtypes = []
+        lib.mpci_ci_basis.restype = C.c_int
+        lib.mpci_ci_basis.argtypes = [C.c_int] + [_lib._dp] * 4
+        lib.mpci_ci_dipoles.restype = C.c_int
+        lib.mpci_ci_dipoles.argtypes = [C.c_int, C.c_int] + [_lib._dp] * 4
+        _ci = lib
+    return _ci
+
+
+def _check(rc):
+    if rc != 0:
+        msg = ci_lib().mpci_ci_last_error()
+        raise _lib.MpciError(rc, msg.decode() if msg else "")
+
+
+def ci_basis(ens, v12_packed):
+    """Eigenvalues (ascending) and eigenvectors (rows) of diag(ens) + V12,
+    V12 given as the packed upper triangle of w2e's V_12 dataset."""
+    ens = np.ascontiguousarray(ens, dtype=np.float64)
+    v12 = np.ascontiguousarray(v12_packed, dtype=np.float64)
+    n = len(ens)
+    if v12.shape != (n * (n + 1) // 2,):
+        raise ValueError("V12 must hold n(n+1)/2 values")
+    eig = np.empty(n)
+    vecs = np.empty((n, n))
+    _check(ci_lib().mpci_ci_basis(n, _lib.dptr(ens), _lib.dptr(v12),
+                                  _lib.dptr(eig), _lib.dptr(vecs)))
+    return eig, vecs
+
+
+def ci_dipoles(D, vecs0, vecs1):
+    """V_L^T D V_{L+1} (row-major), eigenvector matrices with eigenvectors
+    as rows."""
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    n0, n1 = D.shape
+    v0 = np.ascontiguousarray(vecs0, dtype=np.float64)
+    v1 = np.ascontiguousarray(vecs1, dtype=np.float64)
+    if v0.shape != (n0, n0) or v1.shape != (n1, n1):
+        raise ValueError("eigenvector matrices must match the dipole block")
+    out = np.empty((n0, n1))
+    _check(ci_lib().mpci_ci_dipoles(n0, n1, _lib.dptr(D), _lib.dptr(v0),
+                                    _lib.dptr(v1), _lib.dptr(out)))
+    return out
