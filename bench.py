@@ -17,7 +17,8 @@ kernels inside graph replays).
 Extra figures on the same line: the reference mode (fixed dt = 0.1, field
 frozen per step, as src/tdse/tdse.cpp:186-204), the batch of 64 pulses on
 FP64 MFMA, the uneven multi-instance batch (32 H-shaped + 32 He pulses),
-the discrete adjoint, and the CPU baseline (the oracle's restatement of the
+the discrete adjoint, the CI basis producer (w2e stage), the H-shaped
+reference-mode case, and the CPU baseline (the oracle's restatement of the
 reference with MKL cblas_dgemv, timed on the host cores over a bounded
 sample).
 """
@@ -214,6 +215,71 @@ def run_multi(mpci, synth, sysg, P=32, attempts=120):
             "mfma_tflops_wall": tf, "mfma_frac_wall": tf / 78.6}
 
 
+S_C1 = [198] * 4
+
+
+def c1_inputs(mpci, synth):
+    """C1: H-shaped synthetic system (4 blocks of 198 states, E ~ U[-0.5,
+    5]), velocity gauge, w = 0.5, E0 = 0.05, 10 cycles, reference mode at
+    dt = 0.1 over the whole pulse (steps = int(tau / dt) as
+    src/tdse/tdse_main.cpp:164)."""
+    E, D = synth.system_arrays(S_C1, seed=synth.HE_SEED + 2, e_lo=-0.5,
+                               e_hi=5.0)
+    w, e0, cyc, dt = 0.5, 0.05, 10, 0.1
+    steps = int(mpci.period(w, cyc) / dt)
+    return E, D, w, e0, cyc, dt, steps
+
+
+def run_c1(mpci, synth, c1):
+    """C1 on the GPU: the full reference-mode pulse (fixed dt, field frozen
+    per step), launch-bound at this size (every D_L is cache-resident)."""
+    import numpy as np
+    E, D, w, e0, cyc, dt, steps = c1
+    g = mpci.System(E, D, gauge="v")
+    pulse = mpci.Pulse.sine(e0 ** 2, w, cyc, gauge="v")
+    F, t = np.empty((1, steps)), 0.0
+    for k in range(steps):
+        F[0, k] = pulse(t + dt)
+        t += dt
+    prop = make_prop(mpci, g)
+    x0 = synth.ground_state(g.N)
+    prop.set_state(x0)
+    prop.prop_fixed(0.0, dt, F[:, :20])
+    prop.set_state(x0)
+    t0 = time.perf_counter()
+    prop.prop_fixed(0.0, dt, F)
+    wall = time.perf_counter() - t0
+    return {"steps": steps, "steps_per_s": round(steps / wall, 1),
+            "bound": "launch/latency: 0.94 MB of dipoles stay in cache, so "
+                     "no HBM roofline applies"}
+
+
+S_W2E = [3000, 3000]
+
+
+def run_w2e(mpci, synth, arrays):
+    """CI basis producer (the w2e stage) on the GPU: two L blocks of 3000
+    product states (dsyevd each) and the dipole pair between them, from host
+    arrays to host arrays (the stage reads and writes files)."""
+    from mpci import ci
+    ens, v12, dip = arrays
+    ci.ci_basis(ens[0][:64], v12[0][:64 * 65 // 2])  # warm the libraries
+    t0 = time.perf_counter()
+    V = []
+    for L in range(len(ens)):
+        V.append(ci.ci_basis(ens[L], v12[L])[1])
+    t_basis = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    ci.ci_dipoles(dip[0], V[0], V[1])
+    t_dip = time.perf_counter() - t1
+    n = S_W2E[0]
+    return {"sizes": S_W2E, "basis_s": round(t_basis, 3),
+            "dipoles_s": round(t_dip, 4),
+            "total_s": round(t_basis + t_dip, 3),
+            "dipole_gemm_tflops": round(
+                2 * 2.0 * n * n * n / t_dip / 1e12, 2)}
+
+
 def run_adjoint(mpci, synth, sysg, B=16, K=256, cycles=20, w=0.7):
     """Discrete-adjoint gradients of the final ground-state population with
     respect to K piecewise-linear envelope samples (uniform in [0, 0.04]
@@ -234,42 +300,111 @@ def run_adjoint(mpci, synth, sysg, B=16, K=256, cycles=20, w=0.7):
             "adjoint_over_forward": st.bwd_ms / st.fwd_ms}
 
 
-def cpu_baseline(E, D, budget_s=12.0):
+def host_cores():
+    """(logical CPUs, physical cores) of the host, the latter from lscpu."""
+    import subprocess
+    logical = os.cpu_count() or 1
+    try:
+        out = subprocess.run(["lscpu", "-p=Core,Socket"], capture_output=True,
+                             text=True, timeout=10).stdout
+        pairs = {ln for ln in out.splitlines() if ln and not ln.startswith("#")}
+        physical = len(pairs) or None
+    except (OSError, subprocess.SubprocessError):
+        physical = None
+    return logical, physical
+
+
+def mkl_set_threads(path, n):
+    import ctypes
+    try:
+        ctypes.CDLL(path).MKL_Set_Num_Threads(ctypes.c_int(n))
+        return True
+    except (OSError, AttributeError):
+        return False
+
+
+def cpu_baseline(E, D, c1, w2e=None, budget_s=12.0):
     """The oracle (reference call pattern: 4 cblas_dgemv per block per RHS,
-    stride 2, boost do_step stage algebra) with MKL on the host cores."""
+    stride 2, boost do_step stage algebra) with MKL on the host cores, then
+    on one thread; and the C1 H-shaped reference-mode pulse (c1 = its system
+    arrays, pulse and step table) at both thread counts."""
     import numpy as np
     from oracle import oracle as orc
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") \
         else (os.cpu_count() or 1)
     cores = min(cores, 16)
     os.environ.setdefault("MKL_NUM_THREADS", str(cores))
-    blas = "builtin loops (1 thread)"
+    blas, mkl = "builtin loops (1 thread)", None
     for path in ("/opt/conda/lib/libmkl_rt.so.1", "/opt/conda/lib/libmkl_rt.so"):
         if os.path.exists(path) and orc.use_blas(path):
-            blas = f"MKL cblas_dgemv ({path})"
+            blas, mkl = f"MKL cblas_dgemv ({path})", path
             break
     else:
         cores = 1
+    if mkl:
+        mkl_set_threads(mkl, cores)
     osys = orc.System(E, D, gauge="v")
     pulse = orc.Pulse.sine(E0_AU ** 2, W_AU, CYCLES, gauge="v")
     tau = orc.period(W_AU, CYCLES)
     x = np.zeros(2 * osys.N)
     x[0] = 1.0
-    # one step to size the sample, then as many attempts as fit the budget
-    t0 = time.perf_counter()
-    osys.step_stages(np.zeros(13), x, 0.05)
-    one = time.perf_counter() - t0
-    n = max(2, min(50, int(budget_s / max(one, 1e-3))))
-    t0 = time.perf_counter()
-    _, sto, _ = osys.prop_adaptive(pulse, 0.0, tau, 0.05, TOL, TOL, x,
-                                   max_attempts=n, allow_limit=True)
-    acc = sto.accepted
-    wall = time.perf_counter() - t0
+
+    def sample(budget, cap):
+        # one step to size the sample, then as many attempts as fit
+        t0 = time.perf_counter()
+        osys.step_stages(np.zeros(13), x, 0.05)
+        one = time.perf_counter() - t0
+        n = max(2, min(cap, int(budget / max(one, 1e-3))))
+        t0 = time.perf_counter()
+        _, sto, _ = osys.prop_adaptive(pulse, 0.0, tau, 0.05, TOL, TOL, x,
+                                       max_attempts=n, allow_limit=True)
+        return n, sto.accepted, time.perf_counter() - t0
+
+    n, acc, wall = sample(budget_s, 50)
+    out = {"value": round(acc / wall, 3), "unit": "steps/s", "cores": cores,
+           "kind": "port",
+           "sample": f"first {n} adaptive RKF78 attempts of the same He pulse "
+                     f"(tol 1e-10) with the oracle, {blas}, {wall:.1f} s"}
+    logical, physical = host_cores()
+    out["host_logical_cpus"] = logical
+    out["host_physical_cores"] = physical
+    if mkl and mkl_set_threads(mkl, 1):
+        n1, acc1, wall1 = sample(budget_s / 2, 10)
+        out["value_1thread"] = round(acc1 / wall1, 3)
+        out["sample_1thread"] = f"first {n1} attempts, {wall1:.1f} s"
+        if acc1 / wall1 >= 0.9 * out["value"]:
+            # the reference's stride-2 cblas_dgemv calls do not use MKL's
+            # threads: the sample ran on one core whatever was allowed
+            out["sample"] += (f"; MKL allowed {cores} threads, but the "
+                              "stride-2 dgemv of the reference's call pattern "
+                              "runs on one (the 1-thread rate is the same)")
+        mkl_set_threads(mkl, cores)
+    # C1: the H-shaped reference-mode pulse, cache-resident (0.94 MB of D)
+    Eh, Dh, w, e0, cyc, dt, steps = c1
+    oh = orc.System(Eh, Dh, gauge="v")
+    ph = orc.Pulse.sine(e0 ** 2, w, cyc, gauge="v")
+    xh = np.zeros(2 * oh.N)
+    xh[0] = 1.0
+    for thr in ((cores, 1) if mkl else (1,)):
+        if mkl:
+            mkl_set_threads(mkl, thr)
+        t0 = time.perf_counter()
+        oh.prop_fixed(ph, 0.0, dt, steps, xh)
+        out[f"c1_h4x198_steps_per_s_{thr}thr"] = round(
+            steps / (time.perf_counter() - t0), 1)
+    if mkl:
+        mkl_set_threads(mkl, cores)
     orc.use_builtin_dgemv()
-    return {"value": round(acc / wall, 3), "unit": "steps/s", "cores": cores,
-            "kind": "port",
-            "sample": f"first {n} adaptive RKF78 attempts of the same He pulse "
-                      f"(tol 1e-10) with the oracle, {blas}, {wall:.1f} s"}
+    if w2e is not None:
+        # the w2e stage on the CPU restatement (numpy LAPACK/BLAS, all host
+        # threads numpy uses)
+        from oracle import ci_oracle as co
+        ens, v12, dip = w2e
+        t0 = time.perf_counter()
+        V = [co.ci_basis(ens[L], v12[L])[1] for L in range(len(ens))]
+        co.ci_dipoles(dip[0], V[0], V[1])
+        out["w2e_3000x2_s"] = round(time.perf_counter() - t0, 3)
+    return out
 
 
 def copy_calibration(n_bytes=1 << 30, reps=3):
@@ -305,6 +440,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--no-reference-mode", action="store_true")
+    ap.add_argument("--no-w2e", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every attempt eagerly instead of replaying "
                          "the captured step graph (for tools that cannot "
@@ -386,11 +522,20 @@ def main():
         out["adjoint"] = {k: (round(v, 4) if isinstance(v, float) else v)
                           for k, v in a.items()}
         out["adjoint_over_forward"] = round(a["adjoint_over_forward"], 4)
+    c1 = c1_inputs(mpci, synth)
+    w2e = None
+    if not args.no_w2e:
+        log("bench: w2e CI basis (2 x 3000) ...")
+        w2e = synth.w2e_arrays(S_W2E)
+        out["w2e_ci_basis"] = run_w2e(mpci, synth, w2e)
+    if not args.no_reference_mode:
+        log("bench: C1 (H 4 x 198, reference mode, full pulse) ...")
+        out["c1_h4x198"] = run_c1(mpci, synth, c1)
     if args.calibrate_copy:
         out["copy_calibration"] = copy_calibration()
     if not args.no_cpu_baseline:
         log("bench: CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline(E, D)
+        out["cpu_baseline"] = cpu_baseline(E, D, c1, w2e)
     print(json.dumps(out), flush=True)
 
 

@@ -1,5 +1,11 @@
 """Python side of the CI basis producer (the w2e stage) over the C ABI in
-include/mpci_ci.h; the arithmetic runs in lib/libmpci_ci.so on the GPU."""
+include/mpci_ci.h; the arithmetic runs in lib/libmpci_ci.so on the GPU.
+
+lib/libmpci_ci.so links ROCm's rocSOLVER and rocBLAS.  The torch wheel
+carries its own copies of those libraries under the same sonames (another
+ROCm release); in a process that imported torch first, they would be bound
+instead.  Call this module from a process without torch (the `w2e` program
+is one)."""
 from __future__ import annotations
 
 import ctypes as C

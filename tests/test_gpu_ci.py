@@ -21,12 +21,37 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "mp-citdse_amd", "bin")
 
 
-@pytest.fixture(scope="module")
-def ci():
-    import mpci
-    from mpci import ci as c
-    assert mpci.device_count() >= 1
-    return c
+# The CI library links ROCm's rocSOLVER and rocBLAS.  A process that has
+# imported torch already holds torch's own copies of those libraries (same
+# sonames, another ROCm release), so the library calls run in a child
+# process that imports only the mpci package.
+CHILD = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+from mpci import ci
+inp = np.load(sys.argv[2])
+out = {}
+for key in sorted({k.split(":")[0] for k in inp.files}):
+    if key.startswith("basis"):
+        w, V = ci.ci_basis(inp[key + ":ens"], inp[key + ":v12"])
+        out[key + ":w"], out[key + ":V"] = w, V
+    else:
+        out[key + ":out"] = ci.ci_dipoles(inp[key + ":D"], inp[key + ":V0"],
+                                          inp[key + ":V1"])
+np.savez(sys.argv[3], **out)
+"""
+
+
+def gpu_ci(tmp_path, inputs):
+    """Runs ci_basis / ci_dipoles on the GPU in a child process."""
+    import sys
+    fi, fo = tmp_path / "ci_in.npz", tmp_path / "ci_out.npz"
+    np.savez(fi, **inputs)
+    r = subprocess.run([sys.executable, "-c", CHILD,
+                        os.path.join(ROOT, "mp-citdse_amd"), str(fi), str(fo)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return dict(np.load(fo))
 
 
 def pack_upper(H):
@@ -37,19 +62,23 @@ def gap(w):
     return np.diff(w).min() if len(w) > 1 else 1.0
 
 
-@pytest.mark.parametrize("n", [1, 2, 7, 64, 301])
-def test_ci_basis_vs_oracle(ci, n):
+def test_ci_basis_vs_oracle(tmp_path):
     from oracle import ci_oracle as co
-    rng = np.random.default_rng(n)
-    ens = np.sort(rng.uniform(-2.9, 5, n))
-    A = rng.standard_normal((n, n)) * (0.2 / math.sqrt(n))
-    v12 = pack_upper(A + A.T)
-    w, V = ci.ci_basis(ens, v12)
-    wo, Vo = co.ci_basis(ens, v12)
-    scale = max(1.0, np.abs(wo).max())
-    assert np.abs(w - wo).max() < 1e-12 * scale
-    assert np.abs(V - Vo).max() < max(1e-9, 1e-14 / gap(wo))
-    assert np.abs(V @ V.T - np.eye(n)).max() < 1e-12
+    inputs, cases = {}, {}
+    for n in (1, 2, 7, 64, 301):
+        rng = np.random.default_rng(n)
+        ens = np.sort(rng.uniform(-2.9, 5, n))
+        A = rng.standard_normal((n, n)) * (0.2 / math.sqrt(n))
+        v12 = pack_upper(A + A.T)
+        inputs[f"basis{n}:ens"], inputs[f"basis{n}:v12"] = ens, v12
+        cases[n] = co.ci_basis(ens, v12)
+    out = gpu_ci(tmp_path, inputs)
+    for n, (wo, Vo) in cases.items():
+        w, V = out[f"basis{n}:w"], out[f"basis{n}:V"]
+        scale = max(1.0, np.abs(wo).max())
+        assert np.abs(w - wo).max() < 1e-12 * scale, n
+        assert np.abs(V - Vo).max() < max(1e-9, 1e-14 / gap(wo)), n
+        assert np.abs(V @ V.T - np.eye(n)).max() < 1e-12, n
 
 
 def sym(rng, n, scale):
@@ -57,7 +86,7 @@ def sym(rng, n, scale):
     return A + A.T
 
 
-def test_ci_dipoles_vs_oracle(ci):
+def test_ci_dipoles_vs_oracle(tmp_path):
     from oracle import ci_oracle as co
     rng = np.random.default_rng(7)
     n0, n1 = 57, 43
@@ -66,7 +95,8 @@ def test_ci_dipoles_vs_oracle(ci):
     V1 = co.ci_basis(np.sort(rng.uniform(-1, 1, n1)),
                      pack_upper(sym(rng, n1, 0.05)))[1]
     D = rng.standard_normal((n0, n1)) / math.sqrt(n1)
-    got = ci.ci_dipoles(D, V0, V1)
+    out = gpu_ci(tmp_path, {"dip:D": D, "dip:V0": V0, "dip:V1": V1})
+    got = out["dip:out"]
     ref = co.ci_dipoles(D, V0, V1)
     assert np.abs(got - ref).max() < 1e-12 * max(1, np.abs(ref).max())
 
