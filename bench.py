@@ -378,6 +378,7 @@ def cpu_baseline(E, D, c1, w2e=None, budget_s=12.0):
             out["sample"] += (f"; MKL allowed {cores} threads, but the "
                               "stride-2 dgemv of the reference's call pattern "
                               "runs on one (the 1-thread rate is the same)")
+            out["cores"] = 1
         mkl_set_threads(mkl, cores)
     # C1: the H-shaped reference-mode pulse, cache-resident (0.94 MB of D)
     Eh, Dh, w, e0, cyc, dt, steps = c1
