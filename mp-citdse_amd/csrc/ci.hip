@@ -1,12 +1,15 @@
-// CI basis producer on the GPU (the w2e stage): symmetric eigendecomposition
-// of the two-electron Hamiltonian per total angular momentum (rocSOLVER
-// dsyevd) and the projection of the product-basis dipoles onto the CI
-// eigenbasis (two rocBLAS dgemm, in the reference's order).  C ABI in
-// include/mpci_ci.h.
+// Producers of the tdse dipole inputs on the GPU.  The w2e stage:
+// symmetric eigendecomposition of the two-electron Hamiltonian per total
+// angular momentum (rocSOLVER dsyevd) and the projection of the
+// product-basis dipoles onto the CI eigenbasis (two rocBLAS dgemm, in the
+// reference's order).  The d1e stage: one-electron dipole blocks by
+// quadrature, a scaled product of the radial-function samples (one fused
+// scaling kernel, one rocBLAS dgemm).  C ABI in include/mpci_ci.h.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -105,9 +108,77 @@ __global__ __launch_bounds__(256) void k_ci_fix_signs(int n,
     for (int r = t; r < n; r += 256) v[r] = -v[r];
 }
 
+// The d1e integrand's second factor times the quadrature weight, per
+// (state, node): B[n][i] = qw_i x_i P(x_i) (length) or
+// qw_i (P'(x_i) - v2 P(x_i) / x_i) (velocity).
+__global__ void k_d1e_weights(int nen, int qsz, int velocity, double v2,
+                              const double *__restrict__ qx,
+                              const double *__restrict__ qw,
+                              const double *__restrict__ w,
+                              const double *__restrict__ wp,
+                              double *__restrict__ B) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)nen * qsz) return;
+  const int i = (int)(e % qsz);
+  const double x = qx[i];
+  B[e] = velocity ? qw[i] * (wp[e] - v2 * w[e] / x) : qw[i] * x * w[e];
+}
+
 }  // namespace
 
 extern "C" const char *mpci_ci_last_error(void) { return g_ci_err.c_str(); }
+
+extern "C" int mpci_d1e_dipoles(int nen, int qsz, int l, char gauge,
+                                const double *qx, const double *qw,
+                                const double *wfn_l, const double *wfn_l1,
+                                const double *wfnp_l1, double *out) {
+  const bool vel = gauge == 'v';
+  if (nen < 1 || qsz < 1 || l < 0 || (gauge != 'v' && gauge != 'l') || !qx ||
+      !qw || !wfn_l || !wfn_l1 || (vel && !wfnp_l1) || !out)
+    return ci_fail(MPCI_ERR_ARG, "mpci_d1e_dipoles: bad argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return ci_fail(MPCI_ERR_HIP, "no GPU: the 1e dipoles are computed on the "
+                                 "GPU only");
+  const long long nq = (long long)nen * qsz;
+  DevMem<double> dx, dw, dW1, dW2, dWp, dB, dO;
+  CI_HIP(dx.alloc(qsz));
+  CI_HIP(dw.alloc(qsz));
+  CI_HIP(dW1.alloc(nq));
+  CI_HIP(dW2.alloc(nq));
+  CI_HIP(dB.alloc(nq));
+  CI_HIP(dO.alloc((long long)nen * nen));
+  CI_HIP(hipMemcpy(dx.p, qx, sizeof(double) * qsz, hipMemcpyHostToDevice));
+  CI_HIP(hipMemcpy(dw.p, qw, sizeof(double) * qsz, hipMemcpyHostToDevice));
+  CI_HIP(hipMemcpy(dW1.p, wfn_l, sizeof(double) * nq, hipMemcpyHostToDevice));
+  CI_HIP(hipMemcpy(dW2.p, wfn_l1, sizeof(double) * nq,
+                   hipMemcpyHostToDevice));
+  if (vel) {
+    CI_HIP(dWp.alloc(nq));
+    CI_HIP(hipMemcpy(dWp.p, wfnp_l1, sizeof(double) * nq,
+                     hipMemcpyHostToDevice));
+  }
+  // the reference's integer expressions (integrator.cpp:9, dmx1e.cpp:72)
+  const int lp1 = l + 1;
+  const double v2 = (l * (l + 1) - lp1 * (lp1 + 1)) * 0.5;
+  const double kl = sqrt((double)(lp1 * lp1) / (4.0 * lp1 * lp1 - 1));
+  hipLaunchKernelGGL(k_d1e_weights, dim3((unsigned)((nq + 255) / 256)),
+                     dim3(256), 0, 0, nen, qsz, vel ? 1 : 0, v2, dx.p, dw.p,
+                     dW2.p, vel ? dWp.p : dW2.p, dB.p);
+  CI_HIP(hipGetLastError());
+  Blas b;
+  CI_BLAS(rocblas_create_handle(&b.h));
+  const double zero = 0.0;
+  // Column-major views of the row-major nen x qsz arrays are their
+  // transposes: out^T (nen x nen) = B W_l^T, i.e. out = k_l W_l B^T.
+  CI_BLAS(rocblas_dgemm(b.h, rocblas_operation_transpose,
+                        rocblas_operation_none, nen, nen, qsz, &kl, dB.p, qsz,
+                        dW1.p, qsz, &zero, dO.p, nen));
+  CI_HIP(hipDeviceSynchronize());
+  CI_HIP(hipMemcpy(out, dO.p, sizeof(double) * nen * nen,
+                   hipMemcpyDeviceToHost));
+  return MPCI_OK;
+}
 
 extern "C" int mpci_ci_basis(int n, const double *ens, const double *v12_packed,
                              double *eig, double *vecs) {

@@ -6,16 +6,21 @@
 //         (w2e.cpp:67-71, 115-120), dat/<pot>0.h5:/En (the 1e threshold
 //         pes reads, pes.cpp:168-176) and placeholder dat/cfg-<L>.inp files
 //         (tdse copies them, tdse_main.cpp:122-126).
-//   -w:   instead of the CI files, the product-basis inputs the w2e stage
-//         reads (w2e.cpp:25-42, 93-102): dat/<pot>2_<L>En.h5:/e_2e,
+//   -w:   with -e 2, instead of the CI files, the product-basis inputs the
+//         w2e stage reads (w2e.cpp:25-42, 93-102): dat/<pot>2_<L>En.h5:/e_2e,
 //         dat/<pot>V12_<L>.h5:/V_12 (packed upper triangle) and
-//         dat/<pot>2_<L><L+1><g>.h5:/d_if, plus the threshold and cfg files.
+//         dat/<pot>2_<L><L+1><g>.h5:/d_if, plus the threshold and cfg files;
+//         with -e 1, the radial functions the d1e stage reads
+//         (dmx1e.cpp:34-62): dat/<pot>_w1e<l>.h5 with /Pr_o and /Pr_p
+//         (n x q samples and derivatives, every -s size equal) and /Qr_o,
+//         /Qw_o (q quadrature nodes and weights, -q q), plus
+//         dat/<pot><l>.h5:/En energies for the later tdse run.
 // The values come from host/synth.c, so `mpci.synth.system_arrays` (and
 // `mpci.synth.w2e_arrays` for -w) with the same sizes and seed reproduces
 // them bit for bit.
 //
 // usage: tdse_gen -e 1|2 -p <pot> -s n0,n1,... -g v|l [-d <dir>]
-//                 [-r <seed>] [-t <threshold>] [-w]
+//                 [-r <seed>] [-t <threshold>] [-w] [-q <quadrature size>]
 #include <unistd.h>
 
 #include <algorithm>
@@ -62,6 +67,22 @@ int mpci_synth_w2e(uint64_t seed, int nb, const int *s,
   return 0;
 }
 
+// Seeded d1e inputs: quadrature nodes x ~ U[0.05, 60] sorted and weights
+// w ~ U[0, 0.2] (streams 9000, 9001); per l, function samples and
+// derivative samples ~ N(0, 0.1^2) (streams 8000 + l, 8500 + l).
+void mpci_synth_w1e_quadrature(uint64_t seed, int qsz, double *qx,
+                               double *qw) {
+  mpci_synth_uniform(seed, 9000, qsz, 0.05, 60.0, qx);
+  std::sort(qx, qx + qsz);
+  mpci_synth_uniform(seed, 9001, qsz, 0.0, 0.2, qw);
+}
+
+void mpci_synth_w1e_functions(uint64_t seed, int l, int nen, int qsz,
+                              double *po, double *pp) {
+  mpci_synth_normal(seed, 8000 + l, (long long)nen * qsz, 0.1, po);
+  mpci_synth_normal(seed, 8500 + l, (long long)nen * qsz, 0.1, pp);
+}
+
 }  // namespace
 
 int main(int argc, char *argv[]) {
@@ -71,11 +92,13 @@ int main(int argc, char *argv[]) {
   uint64_t seed = 20261202;
   double threshold = -2.0;
   bool w2e_inputs = false;
+  int qsz = 64;
   for (;;) {
-    int c = getopt(argc, argv, "e:p:s:g:d:r:t:w");
+    int c = getopt(argc, argv, "e:p:s:g:d:r:t:wq:");
     if (c == -1) break;
     switch (c) {
     case 'w': w2e_inputs = true; break;
+    case 'q': qsz = std::atoi(optarg); break;
     case 'e': e_num = std::atoi(optarg); break;
     case 'p': pot = optarg; break;
     case 's': sizes = optarg; break;
@@ -104,6 +127,45 @@ int main(int argc, char *argv[]) {
       return 2;
     }
   const int nb = (int)s.size();
+  if (w2e_inputs && e_num == 1) {
+    for (int n : s)
+      if (n != s[0]) {
+        std::cerr << "tdse_gen: -e 1 -w needs equal sizes (one basis size "
+                     "for every l)\n";
+        return 2;
+      }
+    if (qsz < 1) {
+      std::cerr << "tdse_gen: -q must be >= 1\n";
+      return 2;
+    }
+    try {
+      std::filesystem::create_directories(dir);
+      const std::string base = dir + "/" + pot;
+      const int nen = s[0];
+      std::vector<double> qx(qsz), qw(qsz);
+      mpci_synth_w1e_quadrature(seed, qsz, qx.data(), qw.data());
+      std::vector<double> po((size_t)nen * qsz), pp((size_t)nen * qsz),
+          en(nen);
+      for (int l = 0; l < nb; ++l) {
+        mpci_synth_w1e_functions(seed, l, nen, qsz, po.data(), pp.data());
+        std::vector<mpci::H5Out> sets = {{"Pr_o", po.data(), {nen, qsz}},
+                                         {"Pr_p", pp.data(), {nen, qsz}}};
+        if (l == 0) {
+          sets.push_back({"Qr_o", qx.data(), {qsz}});
+          sets.push_back({"Qw_o", qw.data(), {qsz}});
+        }
+        mpci::h5_write_all(base + "_w1e" + std::to_string(l) + ".h5", sets);
+        mpci_synth_uniform(seed, 9500 + l, nen, -0.5, 5.0, en.data());
+        std::sort(en.begin(), en.end());
+        mpci::h5_write_1d(base + std::to_string(l) + ".h5", "En", en.data(),
+                          nen);
+      }
+    } catch (const std::exception &ex) {
+      std::cerr << "tdse_gen: " << ex.what() << "\n";
+      return 1;
+    }
+    return 0;
+  }
   if (w2e_inputs) {
     try {
       std::filesystem::create_directories(dir);

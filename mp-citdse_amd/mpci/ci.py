@@ -1,5 +1,6 @@
-"""Python side of the CI basis producer (the w2e stage) over the C ABI in
-include/mpci_ci.h; the arithmetic runs in lib/libmpci_ci.so on the GPU.
+"""Python side of the producers of the tdse dipole inputs (the w2e CI basis
+and the d1e one-electron dipoles) over the C ABI in include/mpci_ci.h; the
+arithmetic runs in lib/libmpci_ci.so on the GPU.
 
 lib/libmpci_ci.so links ROCm's rocSOLVER and rocBLAS.  The torch wheel
 carries its own copies of those libraries under the same sonames (another
@@ -16,7 +17,8 @@ import numpy as np
 from . import _lib
 
 CI_LIB = os.path.join(_lib.LIB_DIR, "libmpci_ci.so")
-EXPORTED = ["mpci_ci_last_error", "mpci_ci_basis", "mpci_ci_dipoles"]
+EXPORTED = ["mpci_ci_last_error", "mpci_ci_basis", "mpci_ci_dipoles",
+            "mpci_d1e_dipoles"]
 _ci = None
 
 
@@ -34,6 +36,9 @@ def ci_lib():
         lib.mpci_ci_basis.argtypes = [C.c_int] + [_lib._dp] * 4
         lib.mpci_ci_dipoles.restype = C.c_int
         lib.mpci_ci_dipoles.argtypes = [C.c_int, C.c_int] + [_lib._dp] * 4
+        lib.mpci_d1e_dipoles.restype = C.c_int
+        lib.mpci_d1e_dipoles.argtypes = ([C.c_int, C.c_int, C.c_int, C.c_char]
+                                         + [_lib._dp] * 6)
         _ci = lib
     return _ci
 
@@ -71,4 +76,29 @@ def ci_dipoles(D, vecs0, vecs1):
     out = np.empty((n0, n1))
     _check(ci_lib().mpci_ci_dipoles(n0, n1, _lib.dptr(D), _lib.dptr(v0),
                                     _lib.dptr(v1), _lib.dptr(out)))
+    return out
+
+
+def d1e_dipoles(l, gauge, qx, qw, wfn_l, wfn_l1, wfnp_l1=None):
+    """One-electron dipole block between l and l + 1 (the d1e stage):
+    nen x nen from nen x qsz samples of the radial functions (and, for the
+    velocity gauge, of the derivatives of the l + 1 functions)."""
+    qx = np.ascontiguousarray(qx, dtype=np.float64)
+    qw = np.ascontiguousarray(qw, dtype=np.float64)
+    w0 = np.ascontiguousarray(wfn_l, dtype=np.float64)
+    w1 = np.ascontiguousarray(wfn_l1, dtype=np.float64)
+    nen, qsz = w0.shape
+    if w1.shape != (nen, qsz) or qx.shape != (qsz,) or qw.shape != (qsz,):
+        raise ValueError("function samples must be nen x qsz")
+    wp = None
+    if gauge == "v":
+        wp = np.ascontiguousarray(wfnp_l1, dtype=np.float64)
+        if wp.shape != (nen, qsz):
+            raise ValueError("derivative samples must be nen x qsz")
+    out = np.empty((nen, nen))
+    _check(ci_lib().mpci_d1e_dipoles(nen, qsz, l, gauge.encode(),
+                                     _lib.dptr(qx), _lib.dptr(qw),
+                                     _lib.dptr(w0), _lib.dptr(w1),
+                                     _lib.dptr(wp) if wp is not None else None,
+                                     _lib.dptr(out)))
     return out

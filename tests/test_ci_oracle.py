@@ -86,7 +86,7 @@ def test_ci_library_exports_and_refuses_without_gpu():
     from mpci import ci
     txt = open(os.path.join(ROOT, "include", "mpci_ci.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    names = sorted(set(re.findall(r"\b(mpci_ci_[a-z0-9_]+)\s*\(", txt)))
+    names = sorted(set(re.findall(r"\b(mpci_(?:ci|d1e)_[a-z0-9_]+)\s*\(", txt)))
     assert names == sorted(ci.EXPORTED)
     lib = ctypes.CDLL(ci.CI_LIB)
     for n in names:
@@ -97,3 +97,25 @@ def test_ci_library_exports_and_refuses_without_gpu():
     with pytest.raises(Exception) as e:
         ci.ci_basis([0.0, 1.0], [0.0, 0.1, 0.0])
     assert "GPU" in str(e.value)
+
+
+def hydrogen_on_quadrature(q=200, R=60.0):
+    """Gauss-Legendre nodes and weights on (0, R); P_10 = 2 r e^-r and
+    P_21 = r^2 e^(-r/2) / (2 sqrt 6) with P_21'."""
+    x, w = np.polynomial.legendre.leggauss(q)
+    r, wr = 0.5 * R * (x + 1), 0.5 * R * w
+    p10 = 2 * r * np.exp(-r)
+    p21 = r * r * np.exp(-r / 2) / (2 * math.sqrt(6))
+    dp21 = (2 * r - r * r / 2) * np.exp(-r / 2) / (2 * math.sqrt(6))
+    return r, wr, p10[None, :], p21[None, :], dp21[None, :]
+
+
+def test_d1e_hydrogen_1s_2p_both_gauges():
+    r, w, p10, p21, dp21 = hydrogen_on_quadrature()
+    k0 = 1 / math.sqrt(3)           # k_l of l = 0
+    radial = 128 * math.sqrt(6) / 243  # <1s| r |2p> radial integral
+    Dl = co.d1e_dipoles(0, "l", r, w, p10, p21)
+    assert abs(Dl[0, 0] - k0 * radial) < 1e-12
+    # velocity form: <1s| d/dr + (l+1)/r |2p> = (E_2p - E_1s) <1s| r |2p>
+    Dv = co.d1e_dipoles(0, "v", r, w, p10, p21, dp21)
+    assert abs(Dv[0, 0] - k0 * 0.375 * radial) < 1e-12

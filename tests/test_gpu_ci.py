@@ -1,7 +1,9 @@
-"""GPU parity of the CI basis producer (the w2e stage, src/w2e/w2e.cpp:17-125)
-against the CPU restatement (oracle/ci_oracle.py), through the C ABI and
-through the `w2e` drop-in on reference-layout HDF5 files; then `tdse -e 2`
-on the files it wrote.
+"""GPU parity of the producers of the tdse dipole inputs against the CPU
+restatement (oracle/ci_oracle.py): the CI basis (the w2e stage,
+src/w2e/w2e.cpp:17-125) and the one-electron dipoles (the d1e stage,
+src/d1e/dmx1e.cpp:24-139), through the C ABI and through the `w2e` and `d1e`
+drop-ins on reference-layout HDF5 files; then `tdse` on the files they
+wrote.
 
 Bars: eigenvalues within 1e-12 of the oracle (relative to the spectrum's
 scale); eigenvectors, with the sign convention applied on both sides,
@@ -35,6 +37,13 @@ for key in sorted({k.split(":")[0] for k in inp.files}):
     if key.startswith("basis"):
         w, V = ci.ci_basis(inp[key + ":ens"], inp[key + ":v12"])
         out[key + ":w"], out[key + ":V"] = w, V
+    elif key.startswith("d1e"):
+        g = "v" if key.endswith("v") else "l"
+        wp = inp[key + ":wp"] if g == "v" else None
+        out[key + ":out"] = ci.d1e_dipoles(int(inp[key + ":l"]), g,
+                                           inp[key + ":qx"], inp[key + ":qw"],
+                                           inp[key + ":w0"], inp[key + ":w1"],
+                                           wp)
     else:
         out[key + ":out"] = ci.ci_dipoles(inp[key + ":D"], inp[key + ":V0"],
                                           inp[key + ":V1"])
@@ -198,3 +207,103 @@ def test_w2e_cli_then_tdse(tmp_path, oracle):
     x[0::2], x[1::2] = got[:, 1], got[:, 2]
     # eigenvector sign conventions agree, so the states agree coefficientwise
     assert np.abs(x - xr).max() < 1e-9
+
+
+def test_d1e_vs_oracle_and_hydrogen(tmp_path):
+    """Random samples in both gauges against the oracle (1e-12), and the
+    hydrogen 1s-2p radial dipole in both gauges against its closed form."""
+    from oracle import ci_oracle as co
+    rng = np.random.default_rng(11)
+    nen, q = 37, 129
+    qx = np.sort(rng.uniform(0.05, 60, q))
+    qw = rng.uniform(0, 0.2, q)
+    w0, w1, wp = (rng.standard_normal((nen, q)) * 0.1 for _ in range(3))
+    inputs = {}
+    for l, g in ((0, "l"), (2, "l"), (0, "v"), (3, "v")):
+        k = f"d1e{l}{g}"
+        inputs.update({k + ":l": np.array(l), k + ":qx": qx, k + ":qw": qw,
+                       k + ":w0": w0, k + ":w1": w1, k + ":wp": wp})
+    x, w = np.polynomial.legendre.leggauss(200)
+    r, wr = 30.0 * (x + 1), 30.0 * w
+    p10 = (2 * r * np.exp(-r))[None, :]
+    p21 = (r * r * np.exp(-r / 2) / (2 * math.sqrt(6)))[None, :]
+    dp21 = ((2 * r - r * r / 2) * np.exp(-r / 2) / (2 * math.sqrt(6)))[None, :]
+    for g in "lv":
+        k = f"d1eH{g}"
+        inputs.update({k + ":l": np.array(0), k + ":qx": r, k + ":qw": wr,
+                       k + ":w0": p10, k + ":w1": p21, k + ":wp": dp21})
+    out = gpu_ci(tmp_path, inputs)
+    for l, g in ((0, "l"), (2, "l"), (0, "v"), (3, "v")):
+        ref = co.d1e_dipoles(l, g, qx, qw, w0, w1, wp)
+        got = out[f"d1e{l}{g}:out"]
+        assert np.abs(got - ref).max() < 1e-12 * max(1, np.abs(ref).max())
+    radial = 128 * math.sqrt(6) / 243
+    assert abs(out["d1eHl:out"][0, 0] - radial / math.sqrt(3)) < 1e-12
+    assert abs(out["d1eHv:out"][0, 0] - 0.375 * radial / math.sqrt(3)) < 1e-12
+
+
+@pytest.mark.parametrize("gauge", ["v", "l"])
+def test_d1e_cli_then_tdse(tmp_path, oracle, gauge):
+    """tdse_gen -e 1 -w (radial samples, quadrature, energies) -> d1e (GPU)
+    -> tdse -e 1 (GPU) against the oracle."""
+    from mpci import synth
+    from oracle import ci_oracle as co
+    nen, q, nl = 24, 90, 3
+    subprocess.run([os.path.join(BIN, "tdse_gen"), "-e", "1", "-p", "h",
+                    "-s", ",".join([str(nen)] * nl), "-g", gauge, "-w",
+                    "-q", str(q)], cwd=tmp_path, check=True)
+    (tmp_path / "D.yaml").write_text(textwrap.dedent(f"""\
+        Global_Settings:
+          potential: "h"
+          Outer_quadrature_size: {q}
+          gauge: '{gauge}'
+        Basis_Settings:
+          l_max: {nl - 1}
+        """))
+    r = subprocess.run([os.path.join(BIN, "d1e"), "-f", "D.yaml"],
+                       cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"No. of outer quadrature points:              {q}" in r.stdout
+    qx, qw, po, pp, en = synth.w1e_arrays(nen, q, nl)
+    io = _io()
+    D = []
+    for l in range(nl - 1):
+        ref = co.d1e_dipoles(l, gauge, qx, qw, po[l], po[l + 1], pp[l + 1])
+        got = _read(io, tmp_path / "dat" / f"h{l}{l + 1}{gauge}.h5", "d_if",
+                    (nen, nen))
+        assert np.abs(got - ref).max() < 1e-12 * max(1, np.abs(ref).max())
+        D.append(ref)
+    (tmp_path / "H.yaml").write_text(textwrap.dedent(f"""\
+        Global_Settings:
+          potential: "h"
+          gauge: '{gauge}'
+        Basis_Settings:
+          l_max: {nl - 1}
+        Propagator_Settings:
+          states_in_l: [{",".join([str(nen)] * nl)}]
+          dt: 0.05
+          full_precision: true
+        Field_Parameters:
+          shape: "sine"
+          w: 8.5
+          Io: 1.0e13
+          cepd: 0.0
+          cycles: 1
+        """))
+    r = subprocess.run([os.path.join(BIN, "tdse"), "-e", "1", "-f", "H.yaml",
+                        "-o", "out", "-n", "1", "-l", "0"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = oracle.System(en, D, gauge=gauge)
+    w = 8.5 / 27.211396181
+    steps = int(2 * math.pi / w / 0.05)
+    pulse = oracle.Pulse.sine(1.0e13 * (1.0 / 3.509338) * 1e-16, w, 1,
+                              gauge=gauge)
+    x0 = np.zeros(2 * o.N)
+    x0[0] = 1.0
+    xr, tr, _, _ = o.prop_fixed(pulse, 0.0, 0.05, steps, x0)
+    got = np.loadtxt(tmp_path / "out" / ("h_ct17_%f.dat" % tr), comments="#")
+    x = np.empty(2 * o.N)
+    x[0::2], x[1::2] = got[:, 1], got[:, 2]
+    assert np.abs(x - xr).max() < 1e-10
