@@ -280,6 +280,23 @@ def run_w2e(mpci, synth, arrays):
                 2 * 2.0 * n * n * n / t_dip / 1e12, 2)}
 
 
+D1E_NEN, D1E_Q, D1E_NL = 400, 1000, 3
+
+
+def run_d1e(mpci, arrays):
+    """One-electron dipole producer (the d1e stage) on the GPU: 400 radial
+    functions per l on 1000 quadrature nodes, l = 0..2, velocity gauge, host
+    arrays in and out."""
+    from mpci import ci
+    qx, qw, po, pp, _ = arrays
+    ci.d1e_dipoles(0, "v", qx, qw, po[0][:8], po[1][:8], pp[1][:8])  # warm
+    t0 = time.perf_counter()
+    for l in range(D1E_NL - 1):
+        ci.d1e_dipoles(l, "v", qx, qw, po[l], po[l + 1], pp[l + 1])
+    return {"nen": D1E_NEN, "quadrature": D1E_Q, "l_blocks": D1E_NL,
+            "gauge": "v", "total_s": round(time.perf_counter() - t0, 4)}
+
+
 def run_adjoint(mpci, synth, sysg, B=16, K=256, cycles=20, w=0.7):
     """Discrete-adjoint gradients of the final ground-state population with
     respect to K piecewise-linear envelope samples (uniform in [0, 0.04]
@@ -323,7 +340,7 @@ def mkl_set_threads(path, n):
         return False
 
 
-def cpu_baseline(E, D, c1, w2e=None, budget_s=12.0):
+def cpu_baseline(E, D, c1, w2e=None, d1e=None, budget_s=12.0):
     """The oracle (reference call pattern: 4 cblas_dgemv per block per RHS,
     stride 2, boost do_step stage algebra) with MKL on the host cores, then
     on one thread; and the C1 H-shaped reference-mode pulse (c1 = its system
@@ -405,6 +422,13 @@ def cpu_baseline(E, D, c1, w2e=None, budget_s=12.0):
         V = [co.ci_basis(ens[L], v12[L])[1] for L in range(len(ens))]
         co.ci_dipoles(dip[0], V[0], V[1])
         out["w2e_3000x2_s"] = round(time.perf_counter() - t0, 3)
+    if d1e is not None:
+        from oracle import ci_oracle as co
+        qx, qw, po, pp, _ = d1e
+        t0 = time.perf_counter()
+        for l in range(D1E_NL - 1):
+            co.d1e_dipoles(l, "v", qx, qw, po[l], po[l + 1], pp[l + 1])
+        out["d1e_400x1000_s"] = round(time.perf_counter() - t0, 3)
     return out
 
 
@@ -441,7 +465,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--no-reference-mode", action="store_true")
-    ap.add_argument("--no-w2e", action="store_true")
+    ap.add_argument("--no-w2e", action="store_true",
+                    help="skip the w2e and d1e producer legs")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every attempt eagerly instead of replaying "
                          "the captured step graph (for tools that cannot "
@@ -529,6 +554,11 @@ def main():
         log("bench: w2e CI basis (2 x 3000) ...")
         w2e = synth.w2e_arrays(S_W2E)
         out["w2e_ci_basis"] = run_w2e(mpci, synth, w2e)
+    d1e = None
+    if not args.no_w2e:
+        log("bench: d1e dipoles (400 x 1000, 3 l) ...")
+        d1e = synth.w1e_arrays(D1E_NEN, D1E_Q, D1E_NL)
+        out["d1e_dipoles"] = run_d1e(mpci, d1e)
     if not args.no_reference_mode:
         log("bench: C1 (H 4 x 198, reference mode, full pulse) ...")
         out["c1_h4x198"] = run_c1(mpci, synth, c1)
@@ -536,7 +566,7 @@ def main():
         out["copy_calibration"] = copy_calibration()
     if not args.no_cpu_baseline:
         log("bench: CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline(E, D, c1, w2e)
+        out["cpu_baseline"] = cpu_baseline(E, D, c1, w2e, d1e)
     print(json.dumps(out), flush=True)
 
 
