@@ -196,13 +196,22 @@ int mpci_prop_set_instrument(mpci_prop *prop, int enable);
 int mpci_prop_kernel_time(mpci_prop *prop, double *total_ms,
                           long long *launches, long long *bytes_per_launch,
                           long long *flops_per_launch);
-/* Batched form (P >= 2): the MFMA tile (waves along M and N, each wave a
- * 32 x 32 block), K split, kernel (0: single LDS buffer, 1: double-buffered
- * with 16-byte loads) and K chunk the autotuner chose at mpci_prop_create,
- * and its measured time per coupling evaluation in ms.  Any pointer may be
- * NULL. */
+/* Batched form (P >= 2): the MFMA tile the autotuner chose at
+ * mpci_prop_create -- wm x wn waves per workgroup, each wave fm x fn blocks
+ * of 16 x 16 (a panel of 16 fm wm rows by 16 fn wn real columns) -- its K
+ * split, kernel (0: single LDS buffer, 1: double-buffered with 16-byte
+ * loads through registers, 2: double-buffered with direct-to-LDS loads) and
+ * K chunk, and its measured time per coupling evaluation in ms.
+ * Any pointer may be NULL. */
 int mpci_prop_gemm_config(mpci_prop *prop, int *wm, int *wn, int *ks,
-                          double *ms, int *kind, int *tk);
+                          double *ms, int *kind, int *tk, int *fm, int *fn);
+/* The autotuner's candidate tiles: their number, and pinning one of them
+ * (0 <= index < count) on a batched propagator in place of the autotuned
+ * choice (it is timed as the autotuner times it; mpci_prop_gemm_config
+ * then reports it).  For tests and tuning studies; the pin holds for this
+ * propagator only. */
+int mpci_prop_gemm_candidates(int *count);
+int mpci_prop_set_gemm_candidate(mpci_prop *prop, int index);
 
 #ifdef __cplusplus
 }

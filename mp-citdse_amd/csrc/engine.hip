@@ -94,7 +94,7 @@ struct mpci_system {
   // autotuned GEMM tile per batch size P: every propagator of this system
   // with the same P uses the same tiles, hence the same summation order
   struct Tuned {
-    int kind, wm, wn, tk, ks;
+    int kind, wm, wn, fm, fn, tk, ks;
     double ms;
   };
   std::map<int, Tuned> tuned;
@@ -126,7 +126,8 @@ struct mpci_prop {
   hipGraph_t graph = nullptr;
   int graph_track = -1;
   // batched form: chosen MFMA tile (waves along M, N) and K splits
-  int gm_kind = 0, gm_wm = 2, gm_wn = 2, gm_tk = 32, gm_ks = 1;
+  int gm_kind = 0, gm_wm = 2, gm_wn = 2, gm_fm = 2, gm_fn = 2, gm_tk = 32,
+      gm_ks = 1;
   double gemm_ms = 0.0;  // autotuned time of one coupling evaluation
   std::vector<GemmJob> gjobs;
   DevBuf<GemmJob> dgjobs;
@@ -428,12 +429,13 @@ static int ensure_rowmajor(mpci_system *sys) {
   return MPCI_OK;
 }
 
-// GEMM jobs: for every pair and direction, output panels of 32*wm rows x
-// 32*wn real columns, each over one of `ks` equal K ranges.
-static int build_gemm_jobs(mpci_prop *pr, int kind, int wm, int wn, int tk,
-                           int ks) {
+// GEMM jobs: for every pair and direction, output panels of 16*fm*wm rows
+// x 16*fn*wn real columns (wm x wn waves, each fm x fn MFMA blocks of
+// 16 x 16), each over one of `ks` equal K ranges.
+static int build_gemm_jobs(mpci_prop *pr, int kind, int wm, int wn, int fm,
+                           int fn, int tk, int ks) {
   mpci_system *sys = pr->sys;
-  const int TM = 32 * wm, TN = 32 * wn, P2 = 2 * pr->P;
+  const int TM = 16 * fm * wm, TN = 16 * fn * wn, P2 = 2 * pr->P;
   pr->gjobs.clear();
   for (int L = 0; L + 1 < sys->nb; ++L)
     for (int dir = 0; dir < 2; ++dir) {
@@ -462,6 +464,8 @@ static int build_gemm_jobs(mpci_prop *pr, int kind, int wm, int wn, int tk,
   pr->gm_kind = kind;
   pr->gm_wm = wm;
   pr->gm_wn = wn;
+  pr->gm_fm = fm;
+  pr->gm_fn = fn;
   pr->gm_tk = tk;
   pr->gm_ks = ks;
   return MPCI_OK;
@@ -479,12 +483,31 @@ static int launch_gemm(mpci_prop *pr, const double2 *xin,
   hipExtLaunchKernelGGL((k_gemm_coupling<WM, WN>), grid, dim3(64 * WM * WN), \
                         0, pr->stream, e0, e1, 0, pr->dgjobs.p, sys->Dr.p,  \
                         sys->dr_off.p, X, part, pr->desc, pr->ctl.p)
-#define MPCI_GEMM_DB(WM, WN, TK)                                              \
-  hipExtLaunchKernelGGL((k_gemm_db<WM, WN, TK>), grid, dim3(64 * WM * WN),    \
-                        0, pr->stream, e0, e1, 0, pr->dgjobs.p, sys->Dr.p,   \
+#define MPCI_GEMM_DB(WM, WN, TK, FM, FN)                                    \
+  hipExtLaunchKernelGGL((k_gemm_db<WM, WN, TK, FM, FN>), grid,              \
+                        dim3(64 * WM * WN), 0, pr->stream, e0, e1, 0,       \
+                        pr->dgjobs.p, sys->Dr.p, sys->dr_off.p, X, part,    \
+                        pr->desc, pr->ctl.p)
+#define MPCI_GEMM_LDS(WM, WN)                                                \
+  hipExtLaunchKernelGGL((k_gemm_lds<WM, WN>), grid, dim3(64 * WM * WN), 0,   \
+                        pr->stream, e0, e1, 0, pr->dgjobs.p, sys->Dr.p,      \
                         sys->dr_off.p, X, part, pr->desc, pr->ctl.p)
   const int wm = pr->gm_wm, wn = pr->gm_wn, tk = pr->gm_tk;
-  if (pr->gm_kind == 0) {
+  const int f = pr->gm_fm * 10 + pr->gm_fn;
+  if (pr->gm_kind == 2) {
+    if (wm == 2 && wn == 1)
+      MPCI_GEMM_LDS(2, 1);
+    else if (wm == 4 && wn == 1)
+      MPCI_GEMM_LDS(4, 1);
+    else if (wm == 2 && wn == 2)
+      MPCI_GEMM_LDS(2, 2);
+    else if (wm == 4 && wn == 2)
+      MPCI_GEMM_LDS(4, 2);
+    else if (wm == 2 && wn == 4)
+      MPCI_GEMM_LDS(2, 4);
+    else
+      MPCI_GEMM_LDS(4, 4);
+  } else if (pr->gm_kind == 0) {
     if (wm == 2 && wn == 2)
       MPCI_GEMM(2, 2);
     else if (wm == 2 && wn == 4)
@@ -493,89 +516,169 @@ static int launch_gemm(mpci_prop *pr, const double2 *xin,
       MPCI_GEMM(4, 2);
     else
       MPCI_GEMM(4, 4);
+  } else if (f == 44) {
+    if (tk == 32)
+      MPCI_GEMM_DB(2, 2, 32, 4, 4);
+    else if (wm == 2 && wn == 2)
+      MPCI_GEMM_DB(2, 2, 16, 4, 4);
+    else if (wm == 1 && wn == 2)
+      MPCI_GEMM_DB(1, 2, 16, 4, 4);
+    else if (wm == 2 && wn == 1)
+      MPCI_GEMM_DB(2, 1, 16, 4, 4);
+    else
+      MPCI_GEMM_DB(1, 1, 16, 4, 4);
+  } else if (f == 24) {
+    if (wm == 4 && wn == 1)
+      MPCI_GEMM_DB(4, 1, 16, 2, 4);
+    else
+      MPCI_GEMM_DB(2, 2, 16, 2, 4);
+  } else if (f == 42) {
+    MPCI_GEMM_DB(2, 2, 16, 4, 2);
   } else if (tk == 32) {
     if (wm == 2 && wn == 4)
-      MPCI_GEMM_DB(2, 4, 32);
+      MPCI_GEMM_DB(2, 4, 32, 2, 2);
     else
-      MPCI_GEMM_DB(4, 2, 32);
+      MPCI_GEMM_DB(4, 2, 32, 2, 2);
   } else {
     if (wm == 2 && wn == 1)
-      MPCI_GEMM_DB(2, 1, 16);
+      MPCI_GEMM_DB(2, 1, 16, 2, 2);
     else if (wm == 4 && wn == 1)
-      MPCI_GEMM_DB(4, 1, 16);
+      MPCI_GEMM_DB(4, 1, 16, 2, 2);
     else if (wm == 2 && wn == 2)
-      MPCI_GEMM_DB(2, 2, 16);
+      MPCI_GEMM_DB(2, 2, 16, 2, 2);
     else if (wm == 4 && wn == 2)
-      MPCI_GEMM_DB(4, 2, 16);
+      MPCI_GEMM_DB(4, 2, 16, 2, 2);
     else if (wm == 2 && wn == 4)
-      MPCI_GEMM_DB(2, 4, 16);
+      MPCI_GEMM_DB(2, 4, 16, 2, 2);
     else
-      MPCI_GEMM_DB(4, 4, 16);
+      MPCI_GEMM_DB(4, 4, 16, 2, 2);
   }
 #undef MPCI_GEMM
 #undef MPCI_GEMM_DB
+#undef MPCI_GEMM_LDS
   HIPCHK(hipGetLastError());
   return MPCI_OK;
 }
 
-// Autotuner of the batched form: times every (wm, wn, ks) candidate on this
-// system and pulse count (a few coupling evaluations each, HIP events on
-// the propagator's stream) and keeps the fastest.
+// Tile candidates of the batched form.  kind 0: k_gemm_coupling (single LDS
+// buffer, K chunks of 32, 2 x 2 blocks per wave); kind 1: k_gemm_db
+// (double-buffered, 16-byte loads, K chunks of tk, fm x fn blocks per wave);
+// kind 2: k_gemm_lds (double-buffered direct-to-LDS loads, K chunks of 16,
+// 2 x 2 blocks per wave).
+struct GemmCand {
+  int kind, wm, wn, fm, fn, tk, ks;
+};
+static const GemmCand kGemmCands[] = {
+    {0, 2, 2, 2, 2, 32, 1}, {0, 2, 2, 2, 2, 32, 2}, {0, 2, 4, 2, 2, 32, 1},
+    {0, 2, 4, 2, 2, 32, 2}, {0, 4, 2, 2, 2, 32, 1}, {0, 4, 2, 2, 2, 32, 2},
+    {0, 4, 4, 2, 2, 32, 1}, {0, 4, 4, 2, 2, 32, 2}, {0, 4, 4, 2, 2, 32, 4},
+    {0, 2, 4, 2, 2, 32, 4}, {1, 2, 1, 2, 2, 16, 1}, {1, 2, 1, 2, 2, 16, 2},
+    {1, 4, 1, 2, 2, 16, 1}, {1, 4, 1, 2, 2, 16, 2}, {1, 2, 2, 2, 2, 16, 1},
+    {1, 2, 2, 2, 2, 16, 2}, {1, 4, 2, 2, 2, 16, 1}, {1, 4, 2, 2, 2, 16, 2},
+    {1, 2, 4, 2, 2, 16, 1}, {1, 2, 4, 2, 2, 16, 2}, {1, 2, 4, 2, 2, 16, 4},
+    {1, 4, 4, 2, 2, 16, 1}, {1, 4, 4, 2, 2, 16, 2}, {1, 2, 4, 2, 2, 32, 1},
+    {1, 2, 4, 2, 2, 32, 2}, {1, 4, 2, 2, 2, 32, 1}, {1, 4, 2, 2, 2, 32, 2},
+    {1, 4, 2, 2, 2, 16, 4}, {1, 2, 2, 4, 4, 16, 1}, {1, 2, 2, 4, 4, 16, 2},
+    {1, 2, 2, 4, 4, 16, 4}, {1, 1, 2, 4, 4, 16, 1}, {1, 1, 2, 4, 4, 16, 2},
+    {1, 1, 2, 4, 4, 16, 4}, {1, 2, 1, 4, 4, 16, 2}, {1, 2, 1, 4, 4, 16, 4},
+    {1, 1, 1, 4, 4, 16, 2}, {1, 1, 1, 4, 4, 16, 4}, {1, 4, 1, 2, 4, 16, 2},
+    {1, 4, 1, 2, 4, 16, 4}, {1, 2, 2, 2, 4, 16, 2}, {1, 2, 2, 2, 4, 16, 4},
+    {1, 2, 2, 4, 2, 16, 2}, {1, 2, 2, 4, 2, 16, 4}, {1, 2, 2, 4, 4, 32, 2},
+    {1, 2, 2, 4, 4, 32, 4}, {2, 2, 1, 2, 2, 16, 1}, {2, 2, 1, 2, 2, 16, 2},
+    {2, 2, 1, 2, 2, 16, 4}, {2, 4, 1, 2, 2, 16, 1}, {2, 4, 1, 2, 2, 16, 2},
+    {2, 4, 1, 2, 2, 16, 4}, {2, 2, 2, 2, 2, 16, 1}, {2, 2, 2, 2, 2, 16, 2},
+    {2, 2, 2, 2, 2, 16, 4}, {2, 4, 2, 2, 2, 16, 1}, {2, 4, 2, 2, 2, 16, 2},
+    {2, 4, 2, 2, 2, 16, 4}, {2, 2, 4, 2, 2, 16, 1}, {2, 2, 4, 2, 2, 16, 2},
+    {2, 2, 4, 2, 2, 16, 4}, {2, 4, 4, 2, 2, 16, 1}, {2, 4, 4, 2, 2, 16, 2},
+    {2, 4, 4, 2, 2, 16, 4}};
+
+// k_gemm_lds addresses its operands through 32-bit buffer offsets: every
+// D_L copy and the X slab of the propagator must stay below 2 GiB.
+static bool gemm_lds_fits(const mpci_prop *pr) {
+  const mpci_system *sys = pr->sys;
+  const long long lim = 1LL << 31;
+  if ((long long)sys->N * 2 * pr->P * 8 >= lim) return false;
+  for (int L = 0; L + 1 < sys->nb; ++L)
+    if ((long long)sys->s[L] * sys->dr_ld[L] * 8 >= lim) return false;
+  return true;
+}
+
+// Sets up candidate cd and times it: one warm evaluation, then three timed
+// between HIP events on the propagator's stream (all pulses live).
+static int time_gemm_cand(mpci_prop *pr, const GemmCand &cd, hipEvent_t e0,
+                          hipEvent_t e1, float *ms) {
+  TRY(init_desc(pr, cd.ks));
+  TRY(build_gemm_jobs(pr, cd.kind, cd.wm, cd.wn, cd.fm, cd.fn, cd.tk, cd.ks));
+  TRY(launch_gemm(pr, pr->X.p));  // warm
+  HIPCHK(hipEventRecord(e0, pr->stream));
+  for (int r = 0; r < 3; ++r) TRY(launch_gemm(pr, pr->X.p));
+  HIPCHK(hipEventRecord(e1, pr->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipEventElapsedTime(ms, e0, e1));
+  *ms /= 3.0f;
+  return MPCI_OK;
+}
+
+static int set_all_live(mpci_prop *pr, int done) {
+  std::vector<Ctl> c(pr->P);
+  std::memset(c.data(), 0, sizeof(Ctl) * pr->P);
+  for (auto &x : c) x.done = done;
+  return h2d(pr, pr->ctl.p, c.data(), sizeof(Ctl) * pr->P);
+}
+
+// Times every candidate of kGemmCands (or only `only`, when given), sets up
+// the fastest and, for a full search, records it in the system's tune cache
+// for this pulse count.
+static int tune_gemm(mpci_prop *pr, const GemmCand *only) {
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = set_all_live(pr, 0);
+  float best = 1e30f;
+  GemmCand bc = only ? *only : kGemmCands[0];
+  if (rc == MPCI_OK) {
+    if (only) {
+      rc = time_gemm_cand(pr, *only, e0, e1, &best);
+    } else {
+      const bool lds_ok = gemm_lds_fits(pr);
+      for (const GemmCand &cd : kGemmCands) {
+        if (cd.kind == 2 && !lds_ok) continue;
+        float ms = 0.f;
+        rc = time_gemm_cand(pr, cd, e0, e1, &ms);
+        if (rc != MPCI_OK) break;
+        if (ms < best) {
+          best = ms;
+          bc = cd;
+        }
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  TRY(rc);
+  TRY(set_all_live(pr, 1));
+  TRY(init_desc(pr, bc.ks));
+  TRY(build_gemm_jobs(pr, bc.kind, bc.wm, bc.wn, bc.fm, bc.fn, bc.tk, bc.ks));
+  pr->gemm_ms = best;
+  if (!only)
+    pr->sys->tuned[pr->P] = {bc.kind, bc.wm, bc.wn, bc.fm,  bc.fn,
+                             bc.tk,   bc.ks, pr->gemm_ms};
+  return MPCI_OK;
+}
+
+// Autotuner of the batched form: times every candidate tile on this system
+// and pulse count and keeps the fastest; later propagators of the same
+// system and pulse count reuse the choice.
 static int autotune_gemm(mpci_prop *pr) {
   auto it = pr->sys->tuned.find(pr->P);
   if (it != pr->sys->tuned.end()) {
     const auto &t = it->second;
     TRY(init_desc(pr, t.ks));
-    TRY(build_gemm_jobs(pr, t.kind, t.wm, t.wn, t.tk, t.ks));
+    TRY(build_gemm_jobs(pr, t.kind, t.wm, t.wn, t.fm, t.fn, t.tk, t.ks));
     pr->gemm_ms = t.ms;
     return MPCI_OK;
   }
-  // kind 0: k_gemm_coupling (single LDS buffer, K chunks of 32);
-  // kind 1: k_gemm_db (double-buffered, 16-byte loads, K chunks of tk)
-  struct Cand {
-    int kind, wm, wn, tk, ks;
-  };
-  const Cand cands[] = {
-      {0, 2, 2, 32, 1}, {0, 2, 2, 32, 2}, {0, 2, 4, 32, 1}, {0, 2, 4, 32, 2},
-      {0, 4, 2, 32, 1}, {0, 4, 2, 32, 2}, {0, 4, 4, 32, 1}, {0, 4, 4, 32, 2},
-      {0, 4, 4, 32, 4}, {0, 2, 4, 32, 4},
-      {1, 2, 1, 16, 1}, {1, 2, 1, 16, 2}, {1, 4, 1, 16, 1}, {1, 4, 1, 16, 2},
-      {1, 2, 2, 16, 1}, {1, 2, 2, 16, 2}, {1, 4, 2, 16, 1}, {1, 4, 2, 16, 2},
-      {1, 2, 4, 16, 1}, {1, 2, 4, 16, 2}, {1, 2, 4, 16, 4}, {1, 4, 4, 16, 1},
-      {1, 4, 4, 16, 2}, {1, 2, 4, 32, 1}, {1, 2, 4, 32, 2}, {1, 4, 2, 32, 1},
-      {1, 4, 2, 32, 2}, {1, 4, 2, 16, 4}};
-  std::vector<Ctl> c(pr->P);
-  std::memset(c.data(), 0, sizeof(Ctl) * pr->P);  // all pulses live
-  TRY(h2d(pr, pr->ctl.p, c.data(), sizeof(Ctl) * pr->P));
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  float best = 1e30f;
-  Cand bc = cands[0];
-  for (const Cand &cd : cands) {
-    TRY(init_desc(pr, cd.ks));
-    TRY(build_gemm_jobs(pr, cd.kind, cd.wm, cd.wn, cd.tk, cd.ks));
-    TRY(launch_gemm(pr, pr->X.p));  // warm
-    HIPCHK(hipEventRecord(e0, pr->stream));
-    for (int r = 0; r < 3; ++r) TRY(launch_gemm(pr, pr->X.p));
-    HIPCHK(hipEventRecord(e1, pr->stream));
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best) {
-      best = ms;
-      bc = cd;
-    }
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  for (auto &x : c) x.done = 1;
-  TRY(h2d(pr, pr->ctl.p, c.data(), sizeof(Ctl) * pr->P));
-  TRY(init_desc(pr, bc.ks));
-  TRY(build_gemm_jobs(pr, bc.kind, bc.wm, bc.wn, bc.tk, bc.ks));
-  pr->gemm_ms = best / 3.0;
-  pr->sys->tuned[pr->P] = {bc.kind, bc.wm, bc.wn, bc.tk, bc.ks, pr->gemm_ms};
-  return MPCI_OK;
+  return tune_gemm(pr, nullptr);
 }
 
 extern "C" int mpci_prop_create(mpci_system *sys, int P, mpci_prop **out) {
@@ -1480,7 +1583,8 @@ extern "C" int mpci_prop_kernel_time(mpci_prop *pr, double *total_ms,
 }
 
 extern "C" int mpci_prop_gemm_config(mpci_prop *pr, int *wm, int *wn, int *ks,
-                                     double *ms, int *kind, int *tk) {
+                                     double *ms, int *kind, int *tk, int *fm,
+                                     int *fn) {
   if (!pr) return fail(MPCI_ERR_ARG, "NULL argument");
   if (wm) *wm = pr->gm_wm;
   if (wn) *wn = pr->gm_wn;
@@ -1488,5 +1592,30 @@ extern "C" int mpci_prop_gemm_config(mpci_prop *pr, int *wm, int *wn, int *ks,
   if (ms) *ms = pr->gemm_ms;
   if (kind) *kind = pr->gm_kind;
   if (tk) *tk = pr->gm_tk;
+  if (fm) *fm = pr->gm_fm;
+  if (fn) *fn = pr->gm_fn;
   return MPCI_OK;
+}
+
+extern "C" int mpci_prop_gemm_candidates(int *count) {
+  if (!count) return fail(MPCI_ERR_ARG, "NULL argument");
+  *count = (int)(sizeof(kGemmCands) / sizeof(kGemmCands[0]));
+  return MPCI_OK;
+}
+
+extern "C" int mpci_prop_set_gemm_candidate(mpci_prop *pr, int index) {
+  if (!pr) return fail(MPCI_ERR_ARG, "NULL argument");
+  const int n = (int)(sizeof(kGemmCands) / sizeof(kGemmCands[0]));
+  if (index < 0 || index >= n)
+    return fail(MPCI_ERR_ARG, "GEMM candidate index out of range");
+  if (pr->P < 2 || pr->sys->nb < 2)
+    return fail(MPCI_ERR_ARG, "this propagator has no batched coupling");
+  if (kGemmCands[index].kind == 2 && !gemm_lds_fits(pr))
+    return fail(MPCI_ERR_ARG, "candidate needs buffers below 2 GiB");
+  HIPCHK(hipSetDevice(pr->sys->device));
+  // the captured step refers to the job list and partial buffers, which
+  // are reallocated here
+  HIPCHK(hipStreamSynchronize(pr->stream));
+  drop_graph(pr);
+  return tune_gemm(pr, &kGemmCands[index]);
 }

@@ -503,12 +503,12 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_coupling(
 // padding and the ragged edges read as zeros), so one barrier per chunk
 // separates the phases.
 // ---------------------------------------------------------------------
-template <int WM, int WN, int TK>
+template <int WM, int WN, int TK, int FM, int FN>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_db(
     const GemmJob *__restrict__ jobs, const double *__restrict__ Dr,
     const long long *__restrict__ dr_off, const double *__restrict__ X,
     double *__restrict__ part, SysDesc d, const Ctl *__restrict__ ctl) {
-  constexpr int TM = 32 * WM, TN = 32 * WN;
+  constexpr int TM = 16 * FM * WM, TN = 16 * FN * WN;
   constexpr int NTH = 64 * WM * WN;
   constexpr int LDA = TM + 2, LDB = TN + 2;  // padded, even LDS rows
   constexpr int NPA = TM * TK / 2, NPB = TK * TN / 2;  // 16-byte pairs
@@ -608,11 +608,11 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_db(
   const int lane = t & 63, w = t >> 6;
   const int wm = w % WM, wn = w / WM;
   const int li = lane & 15, lk = lane >> 4;
-  dmf4 acc[2][2];
+  dmf4 acc[FM][FN];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < FM; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = dmf4{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < FN; ++b) acc[a][b] = dmf4{0.0, 0.0, 0.0, 0.0};
 
   const int nch = (jb.k1 - jb.k0 + TK - 1) / TK;
   load_chunk(jb.k0);
@@ -630,17 +630,17 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_db(
 #pragma unroll
     for (int ks = 0; ks < TK / 4; ++ks) {
       const int kr = ks * 4 + lk;
-      double a[2], bb[2];
+      double a[FM], bb[FN];
 #pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
-        a[bi] = As[b][kr * LDA + wm * 32 + bi * 16 + li];
+      for (int bi = 0; bi < FM; ++bi)
+        a[bi] = As[b][kr * LDA + wm * 16 * FM + bi * 16 + li];
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-        bb[bj] = Bs[b][kr * LDB + wn * 32 + bj * 16 + li];
+      for (int bj = 0; bj < FN; ++bj)
+        bb[bj] = Bs[b][kr * LDB + wn * 16 * FN + bj * 16 + li];
 #pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
+      for (int bi = 0; bi < FM; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
+        for (int bj = 0; bj < FN; ++bj)
           acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(
               a[bi], bb[bj], acc[bi][bj], 0, 0, 0);
     }
@@ -651,6 +651,171 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_db(
   const long long base =
       jb.dir == 0 ? d.row_part_off[L] : d.col_part_off[L];  // double2 units
   double *out = part + 2 * base + (long long)jb.q * Mdim * P2;
+#pragma unroll
+  for (int bi = 0; bi < FM; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < FN; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = jb.m0 + wm * 16 * FM + bi * 16 + 4 * r + lk;
+        const int n = jb.n0 + wn * 16 * FN + bj * 16 + li;
+        if (m < Mdim && n < P2) out[(long long)m * P2 + n] = acc[bi][bj][r];
+      }
+}
+
+// ---------------------------------------------------------------------
+// Batched coupling on FP64 MFMA with direct-to-LDS operand loads.  Same
+// panels, operand layouts and partial slots as k_gemm_db (wm x wn waves,
+// each 2 x 2 blocks of 16 x 16; K in chunks of 16), but the chunk operands
+// travel HBM -> LDS with buffer_load_dwordx4 ... lds: no VGPR staging, no
+// LDS store instructions, and a lane outside the panel gets an offset
+// beyond the buffer's range, which the hardware returns as zeros.
+//
+// Each load instruction moves 1 KiB: 64 lanes x 16 bytes into consecutive
+// LDS, so an LDS layout is chosen by which global pair each lane fetches.
+// The layouts make every 64-bit fragment read (32 lanes per LDS cycle: 16
+// consecutive m or n at two consecutive k) hit 64 distinct banks:
+//   A, dir 0 (A = D, pairs along k): m-major rows of 16 doubles; row m
+//     holds pair p of the chunk at slot p ^ ((m >> 1) & 7);
+//   A, dir 1 (A = D^T, pairs along m) and B (pairs along n): k-major rows;
+//     row k holds pair p at slot p ^ (8 (k & 1)).
+// Chunk c + 1 is in flight into the other buffer while the MFMAs consume
+// chunk c; the loop is unrolled by two so that each buffer is a distinct
+// LDS object (the compiler then orders reads of one buffer against loads
+// into the other without waits).
+// ---------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+template <int WM, int WN, int DIR>
+__device__ __forceinline__ void gemm_lds_body(
+    const GemmJob &jb, __amdgpu_buffer_rsrc_t rsA, __amdgpu_buffer_rsrc_t rsB,
+    double *__restrict__ out, int Mdim, int P2, double *As0, double *As1,
+    double *Bs0, double *Bs1) {
+  constexpr int TM = 32 * WM, TN = 32 * WN, TK = 16;
+  constexpr int NW = WM * WN;
+  constexpr int NIA = TM * TK / 128, NIB = TK * TN / 128;  // 1 KiB loads
+  static_assert(NIA % NW == 0 && NIB % NW == 0, "loads must split evenly");
+  constexpr int LA = NIA / NW, LB = NIB / NW;  // per wave
+  constexpr unsigned kOut = 0x80000000u;       // beyond any buffer range
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int ld = jb.ld;
+
+  // per load: the byte offset at chunk 0 (k0 = jb.k0), which each chunk
+  // advances by a fixed step, and the K index whose bound k1 decides, per
+  // chunk, whether the lane loads (the other bound is fixed per lane)
+  unsigned a_off[LA], b_off[LB];
+  int a_k[LA], b_k[LB];
+  bool a_in[LA], b_in[LB];
+#pragma unroll
+  for (int u = 0; u < LA; ++u) {
+    const int I = w * LA + u;
+    if (DIR == 0) {
+      const int mrow = 8 * I + (lane >> 3), q = lane & 7;
+      const int gm = jb.m0 + mrow, kk = 2 * (q ^ ((mrow >> 1) & 7));
+      a_in[u] = gm < Mdim;
+      a_k[u] = jb.k0 + kk;
+      a_off[u] = a_in[u] ? (unsigned)((gm * ld + jb.k0 + kk) * 8) : 0u;
+    } else {
+      constexpr int PPR = TM / 2;  // pairs per row
+      const int krow = I * (128 / TM) + lane / PPR, q = lane % PPR;
+      const int gm = jb.m0 + 2 * (q ^ (8 * (krow & 1)));
+      a_in[u] = gm < Mdim;
+      a_k[u] = jb.k0 + krow;
+      a_off[u] = a_in[u] ? (unsigned)(((jb.k0 + krow) * ld + gm) * 8) : 0u;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < LB; ++u) {
+    constexpr int PPR = TN / 2;
+    const int I = w * LB + u;
+    const int krow = I * (128 / TN) + lane / PPR, q = lane % PPR;
+    const int gn = jb.n0 + 2 * (q ^ (8 * (krow & 1)));
+    b_in[u] = gn < P2;
+    b_k[u] = jb.k0 + krow;
+    b_off[u] = b_in[u] ? (unsigned)(((jb.k0 + krow) * P2 + gn) * 8) : 0u;
+  }
+  const unsigned a_step = DIR == 0 ? TK * 8u : (unsigned)(TK * ld * 8);
+  const unsigned b_step = (unsigned)(TK * P2 * 8);
+  // chunk c (k0 = jb.k0 + c TK)
+  auto issue = [&](int c, double *As, double *Bs) {
+#pragma unroll
+    for (int u = 0; u < LA; ++u) {
+      const bool ok = a_in[u] && a_k[u] + c * TK < jb.k1;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rsA, (lds_void_t *)(As + (w * LA + u) * 128), 16,
+          ok ? a_off[u] + c * a_step : kOut, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < LB; ++u) {
+      const bool ok = b_in[u] && b_k[u] + c * TK < jb.k1;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rsB, (lds_void_t *)(Bs + (w * LB + u) * 128), 16,
+          ok ? b_off[u] + c * b_step : kOut, 0, 0, 0);
+    }
+  };
+
+  const int wm = w % WM, wn = w / WM;
+  const int li = lane & 15, lk = lane >> 4;
+  // fragment read offsets (doubles) at k-step 0.  k-major layouts: k-step
+  // ks adds 4 ks rows.  A dir 0: element (m, k) sits at
+  // 16 m + 2 ((k >> 1) ^ sw) + (k & 1), sw = (m >> 1) & 7; the two blocks
+  // bi = 0, 1 (m and m + 16) share sw, and k >> 1 = 2 ks + (lk >> 1)
+  int aoff[2], boff[2];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi) {
+    const int m = wm * 32 + bi * 16 + li;
+    aoff[bi] = DIR == 0 ? m * TK + (lk & 1)
+                        : lk * TM + 2 * ((m >> 1) ^ (8 * (lk & 1))) + (m & 1);
+  }
+  int a_sw = ((li >> 1) & 7) ^ (lk >> 1);  // dir 0: pair slot at k-step 0
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj) {
+    const int n = wn * 32 + bj * 16 + li;
+    boff[bj] = lk * TN + 2 * ((n >> 1) ^ (8 * (lk & 1))) + (n & 1);
+  }
+
+  dmf4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = dmf4{0.0, 0.0, 0.0, 0.0};
+  auto mfma_chunk = [&](const double *As, const double *Bs) {
+    // recomputed per chunk rather than held in registers across the loop
+    asm volatile("" : "+v"(a_sw));
+#pragma unroll
+    for (int ks = 0; ks < TK / 4; ++ks) {
+      double a[2], bb[2];
+      // dir 0: (2 ks + (lk >> 1)) ^ sw = 2 ks ^ a_sw, as 2 ks has no low
+      // bit
+      const int slot = DIR == 0 ? ((2 * ks) ^ a_sw) : 0;
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+        a[bi] = DIR == 0 ? As[aoff[bi] + 2 * slot] : As[aoff[bi] + 4 * ks * TM];
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) bb[bj] = Bs[boff[bj] + 4 * ks * TN];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              a[bi], bb[bj], acc[bi][bj], 0, 0, 0);
+    }
+  };
+
+  const int nch = (jb.k1 - jb.k0 + TK - 1) / TK;
+  if (nch > 0) issue(0, As0, Bs0);
+  __syncthreads();  // chunk 0 landed (each wave waits for its own loads)
+  int c = 0;
+  for (; c + 1 < nch; c += 2) {
+    issue(c + 1, As1, Bs1);
+    mfma_chunk(As0, Bs0);
+    __syncthreads();  // buffer 0 read by all waves; chunk c + 1 landed
+    if (c + 2 < nch) issue(c + 2, As0, Bs0);
+    mfma_chunk(As1, Bs1);
+    __syncthreads();  // buffer 1 read by all waves; chunk c + 2 landed
+  }
+  if (c < nch) mfma_chunk(As0, Bs0);  // odd chunk count: the last one
+
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -663,18 +828,88 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_db(
       }
 }
 
-#define MPCI_GEMM_DB(WM, WN, TK)                                              \
-  template __global__ void k_gemm_db<WM, WN, TK>(                            \
+// eight-wave tiles: registers held to 80 so that three workgroups (the
+// LDS limit, 48 KiB each) share a compute unit
+template <int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) __attribute__((
+    amdgpu_waves_per_eu(WM * WN == 8 ? 6 : 1))) void k_gemm_lds(
+    const GemmJob *__restrict__ jobs, const double *__restrict__ Dr,
+    const long long *__restrict__ dr_off, const double *__restrict__ X,
+    double *__restrict__ part, SysDesc d, const Ctl *__restrict__ ctl) {
+  constexpr int TM = 32 * WM, TN = 32 * WN, TK = 16;
+  constexpr int NTH = 64 * WM * WN;
+  __shared__ __align__(16) double As0[TM * TK], As1[TM * TK];
+  __shared__ __align__(16) double Bs0[TK * TN], Bs1[TK * TN];
+  __shared__ int live;
+
+  const GemmJob jb = jobs[blockIdx.x];
+  const int t = threadIdx.x;
+  const int P2 = 2 * d.P;
+  // skip the panel when every pulse it covers has finished
+  if (t == 0) live = 0;
+  __syncthreads();
+  {
+    const int p0 = jb.n0 >> 1, p1 = min(d.P, (jb.n0 + TN + 1) >> 1);
+    for (int p = p0 + t; p < p1; p += NTH)
+      if (!ctl[p].done) live = 1;
+  }
+  __syncthreads();
+  if (!live) return;
+
+  const int L = jb.L;
+  const int sL = d.off[L + 1] - d.off[L];
+  const int sL1 = d.off[L + 2] - d.off[L + 1];
+  const int Mdim = jb.dir == 0 ? sL : sL1;
+  const int Kdim = jb.dir == 0 ? sL1 : sL;
+  // buffer ranges: the row-major D_L copy (s_L rows of ld doubles) and the
+  // K rows of X the panel multiplies (both below 2 GiB, checked on the host)
+  const __amdgpu_buffer_rsrc_t rsA = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(Dr + dr_off[L]), (short)0, (int)((long long)sL * jb.ld * 8),
+      0x00020000);
+  const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(X + (long long)(jb.dir == 0 ? d.off[L + 1] : d.off[L]) * P2),
+      (short)0, (int)((long long)Kdim * P2 * 8), 0x00020000);
+  const long long base =
+      jb.dir == 0 ? d.row_part_off[L] : d.col_part_off[L];  // double2 units
+  double *out = part + 2 * base + (long long)jb.q * Mdim * P2;
+  if (jb.dir == 0)
+    gemm_lds_body<WM, WN, 0>(jb, rsA, rsB, out, Mdim, P2, As0, As1, Bs0, Bs1);
+  else
+    gemm_lds_body<WM, WN, 1>(jb, rsA, rsB, out, Mdim, P2, As0, As1, Bs0, Bs1);
+}
+
+#define MPCI_GEMM_LDS(WM, WN)                                                 \
+  template __global__ void k_gemm_lds<WM, WN>(                               \
       const GemmJob *, const double *, const long long *, const double *,    \
       double *, SysDesc, const Ctl *);
-MPCI_GEMM_DB(2, 1, 16)
-MPCI_GEMM_DB(4, 1, 16)
-MPCI_GEMM_DB(2, 2, 16)
-MPCI_GEMM_DB(4, 2, 16)
-MPCI_GEMM_DB(2, 4, 16)
-MPCI_GEMM_DB(4, 4, 16)
-MPCI_GEMM_DB(2, 4, 32)
-MPCI_GEMM_DB(4, 2, 32)
+MPCI_GEMM_LDS(2, 1)
+MPCI_GEMM_LDS(4, 1)
+MPCI_GEMM_LDS(2, 2)
+MPCI_GEMM_LDS(4, 2)
+MPCI_GEMM_LDS(2, 4)
+MPCI_GEMM_LDS(4, 4)
+#undef MPCI_GEMM_LDS
+
+#define MPCI_GEMM_DB(WM, WN, TK, FM, FN)                                      \
+  template __global__ void k_gemm_db<WM, WN, TK, FM, FN>(                    \
+      const GemmJob *, const double *, const long long *, const double *,    \
+      double *, SysDesc, const Ctl *);
+MPCI_GEMM_DB(2, 1, 16, 2, 2)
+MPCI_GEMM_DB(4, 1, 16, 2, 2)
+MPCI_GEMM_DB(2, 2, 16, 2, 2)
+MPCI_GEMM_DB(4, 2, 16, 2, 2)
+MPCI_GEMM_DB(2, 4, 16, 2, 2)
+MPCI_GEMM_DB(4, 4, 16, 2, 2)
+MPCI_GEMM_DB(2, 4, 32, 2, 2)
+MPCI_GEMM_DB(4, 2, 32, 2, 2)
+MPCI_GEMM_DB(2, 2, 16, 4, 4)
+MPCI_GEMM_DB(1, 2, 16, 4, 4)
+MPCI_GEMM_DB(2, 1, 16, 4, 4)
+MPCI_GEMM_DB(1, 1, 16, 4, 4)
+MPCI_GEMM_DB(4, 1, 16, 2, 4)
+MPCI_GEMM_DB(2, 2, 16, 2, 4)
+MPCI_GEMM_DB(2, 2, 16, 4, 2)
+MPCI_GEMM_DB(2, 2, 32, 4, 4)
 #undef MPCI_GEMM_DB
 
 template __global__ void k_gemm_coupling<2, 2>(const GemmJob *, const double *,
@@ -817,10 +1052,14 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
           const double2 x = a.X[idx];
           if (s < kStages - 1) {
             double vx = 1.0 * x.x, vy = 1.0 * x.y;
+            // the tableau's zero entries (22 of the 66 below the diagonal
+            // before the last row) skip their loads: a uniform branch on a
+            // constant-memory value
             double2 kj[kStages - 1];
 #pragma unroll
             for (int j = 0; j < kStages - 1; ++j)
-              kj[j] = (j < s) ? a.K[j * NP + idx] : make_double2(0.0, 0.0);
+              kj[j] = (j < s && cA[s + 1][j] != 0.0) ? a.K[j * NP + idx]
+                                                     : make_double2(0.0, 0.0);
 #pragma unroll
             for (int j = 0; j < kStages - 1; ++j) {
               if (j < s) {
@@ -838,9 +1077,13 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
                   make_double2(vx, vy);
           } else {
             double yx = 1.0 * x.x, yy = 1.0 * x.y, ex = 0.0, ey = 0.0;
+            // k_2 .. k_5 carry zero weight in both b and b - bhat
             double2 kj[kStages];
 #pragma unroll
-            for (int j = 0; j < kStages - 1; ++j) kj[j] = a.K[j * NP + idx];
+            for (int j = 0; j < kStages - 1; ++j)
+              kj[j] = (j == 0 || cB[j] != 0.0 || cDB[j] != 0.0)
+                          ? a.K[j * NP + idx]
+                          : make_double2(0.0, 0.0);
             kj[kStages - 1] = k;
             const double2 k1 = kj[0];
 #pragma unroll

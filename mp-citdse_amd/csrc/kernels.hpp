@@ -51,7 +51,11 @@ __global__ void k_relayout_gemv(const double *D, int sL, int sL1, int ncb,
 __global__ void k_gemv_tile(const TileJob *jobs, int njobs, const double *Dg,
                             const double2 *X, double2 *part, SysDesc d,
                             const Ctl *ctl);
-template <int WM, int WN, int TK>
+template <int WM, int WN>
+__global__ void k_gemm_lds(const GemmJob *jobs, const double *Dr,
+                           const long long *dr_off, const double *X,
+                           double *part, SysDesc d, const Ctl *ctl);
+template <int WM, int WN, int TK, int FM, int FN>
 __global__ void k_gemm_db(const GemmJob *jobs, const double *Dr,
                           const long long *dr_off, const double *X,
                           double *part, SysDesc d, const Ctl *ctl);

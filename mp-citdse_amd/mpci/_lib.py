@@ -98,7 +98,10 @@ _SIGS = [
     ("mpci_prop_kernel_time", C.c_int, [_vp, _dp, C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_longlong)]),
-    ("mpci_prop_gemm_config", C.c_int, [_vp, _ip, _ip, _ip, _dp, _ip, _ip]),
+    ("mpci_prop_gemm_config", C.c_int,
+     [_vp, _ip, _ip, _ip, _dp, _ip, _ip, _ip, _ip]),
+    ("mpci_prop_gemm_candidates", C.c_int, [_ip]),
+    ("mpci_prop_set_gemm_candidate", C.c_int, [_vp, C.c_int]),
 ]
 
 # Every symbol the header declares; tests check the library exports them.

@@ -238,15 +238,28 @@ class Propagator:
         return ms.value, n.value, b.value, f.value
 
     def gemm_config(self):
-        wm, wn, ks, kind, tk = (C.c_int(), C.c_int(), C.c_int(), C.c_int(),
-                                C.c_int())
+        wm, wn, ks, kind, tk, fm, fn = (C.c_int() for _ in range(7))
         ms = C.c_double()
         check(_lib.tdse_lib().mpci_prop_gemm_config(
             self.h, C.byref(wm), C.byref(wn), C.byref(ks), C.byref(ms),
-            C.byref(kind), C.byref(tk)))
-        return {"kernel": ("k_gemm_coupling", "k_gemm_db")[kind.value],
-                "wm": wm.value, "wn": wn.value, "tk": tk.value,
-                "ksplit": ks.value, "ms": ms.value}
+            C.byref(kind), C.byref(tk), C.byref(fm), C.byref(fn)))
+        return {"kernel": ("k_gemm_coupling", "k_gemm_db",
+                           "k_gemm_lds")[kind.value],
+                "wm": wm.value, "wn": wn.value, "fm": fm.value,
+                "fn": fn.value, "tk": tk.value, "ksplit": ks.value,
+                "ms": ms.value}
+
+    @staticmethod
+    def gemm_candidates():
+        """Number of candidate tiles the batched-form autotuner searches."""
+        n = C.c_int()
+        check(_lib.tdse_lib().mpci_prop_gemm_candidates(C.byref(n)))
+        return n.value
+
+    def set_gemm_candidate(self, index):
+        """Pin candidate tile `index` in place of the autotuned one."""
+        check(_lib.tdse_lib().mpci_prop_set_gemm_candidate(self.h, int(index)))
+        return self.gemm_config()
 
 
 def multi_adaptive(props, t0, t1, h0, atol, rtol, max_attempts=10_000_000,

@@ -34,8 +34,9 @@ def test_batched_rhs_parity(mpci, oracle, P, gauge):
     g, o = build(mpci, oracle, [70, 130, 45, 90], gauge, seed=P)
     prop = mpci.Propagator(g, P=P)
     cfg = prop.gemm_config()
-    assert cfg["wm"] in (2, 4) and cfg["wn"] in (1, 2, 4) and cfg["ms"] > 0
-    assert cfg["kernel"] in ("k_gemm_coupling", "k_gemm_db")
+    assert cfg["wm"] in (1, 2, 4) and cfg["wn"] in (1, 2, 4) and cfg["ms"] > 0
+    assert cfg["fm"] in (2, 4) and cfg["fn"] in (2, 4)
+    assert cfg["kernel"] in ("k_gemm_coupling", "k_gemm_db", "k_gemm_lds")
     rng = np.random.default_rng(P)
     x = rng.standard_normal((P, 2 * g.N))
     F = rng.uniform(-1, 1, P)
@@ -178,3 +179,48 @@ def test_empty_runs_leave_state(mpci, oracle):
         xr, sr, _ = o.prop_adaptive(po, 0.0, t1[p], 0.05, 1e-10, 1e-10, xs[p])
         assert st[p].accepted == sr.accepted
         assert np.abs(prop.get_state(p) - xr).max() < 1e-10
+
+
+@pytest.mark.parametrize("P", [3, 40])
+def test_every_gemm_candidate(mpci, oracle, P):
+    """Each tile the autotuner may choose, pinned in turn, on blocks that
+    are ragged against every tile height (70, 130, 45, 90 rows) and pulse
+    counts that leave partial column panels (6 and 80 real columns)."""
+    g, o = build(mpci, oracle, [70, 130, 45, 90], "l", seed=P + 100)
+    prop = mpci.Propagator(g, P=P)
+    rng = np.random.default_rng(P)
+    x = rng.standard_normal((P, 2 * g.N))
+    F = rng.uniform(-1, 1, P)
+    ref = np.stack([o.rhs(F[p], x[p]) for p in range(P)])
+    bar = 1e-13 * max(1, np.abs(ref).max())
+    seen = set()
+    for i in range(mpci.Propagator.gemm_candidates()):
+        cfg = prop.set_gemm_candidate(i)
+        seen.add((cfg["kernel"], cfg["wm"], cfg["wn"], cfg["fm"], cfg["fn"],
+                  cfg["tk"], cfg["ksplit"]))
+        got = prop.rhs(F, x.reshape(-1)).reshape(P, -1)
+        assert np.abs(got - ref).max() <= bar, (i, cfg)
+    assert len(seen) == mpci.Propagator.gemm_candidates()
+
+
+def test_pinned_candidate_in_captured_steps(mpci, oracle):
+    """Pinning a tile after steps were captured rebuilds the captured step
+    (the job list and partial buffers are reallocated) and gives the same
+    fixed-step result as the oracle."""
+    from mpci import synth
+    g, o = build(mpci, oracle, [50, 64, 33], "v", seed=22)
+    P, steps, dt = 4, 20, 0.1
+    w, E0, cyc = synth.pulse_batch(P, seed=4)
+    pulses = [oracle.Pulse.sine(E0[p] ** 2, w[p], int(cyc[p]))
+              for p in range(P)]
+    F = np.array([[pl(dt * (k + 1)) for k in range(steps)] for pl in pulses])
+    x0 = synth.ground_state(g.N)
+    prop = mpci.Propagator(g, P=P)
+    for p in range(P):
+        prop.set_state(x0, p)
+    prop.prop_fixed(0.0, dt, F[:, :steps // 2])
+    prop.set_gemm_candidate(mpci.Propagator.gemm_candidates() - 1)
+    prop.prop_fixed(steps // 2 * dt, dt, F[:, steps // 2:])
+    for p in range(P):
+        xr, _, _, _ = o.prop_fixed(pulses[p], 0.0, dt, steps, x0, track=0)
+        assert np.abs(prop.get_state(p) - xr).max() < 1e-12, p
