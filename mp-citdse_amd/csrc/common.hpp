@@ -45,6 +45,9 @@ struct TileJob {
 // X_{L+1}) or transpose (dir 1: rows of block L+1, from X_L).  X is the
 // state viewed as a row-major matrix with 2P real columns.
 constexpr int kGmTK = 32;  // K chunk staged in LDS
+// Lanes per row of k_stage for one pulse (they split a row's partials).
+constexpr int kStageSub1 = 4;
+
 struct GemmJob {
   int L;
   int dir;

@@ -626,15 +626,29 @@ static int set_all_live(mpci_prop *pr, int done) {
   return h2d(pr, pr->ctl.p, c.data(), sizeof(Ctl) * pr->P);
 }
 
+// Stage-side cost of a K split of ks: each split slot is one more partial
+// panel per direction that the GEMM writes and k_stage reads back, at the
+// stage kernel's streaming rate (about 5 TB/s measured).  Added to the
+// candidate's GEMM time when choosing, so that equal GEMM times favour the
+// smaller split.
+static double partial_cost_ms(const mpci_prop *pr, int ks) {
+  const mpci_system *sys = pr->sys;
+  double bytes = 0.0;
+  for (int L = 0; L + 1 < sys->nb; ++L)
+    bytes += (double)(sys->s[L] + sys->s[L + 1]) * 2.0 * pr->P * 8.0;
+  return 2.0 * ks * bytes / 5e12 * 1e3;
+}
+
 // Times every candidate of kGemmCands (or only `only`, when given), sets up
-// the fastest and, for a full search, records it in the system's tune cache
-// for this pulse count.
+// the one with the least GEMM time plus partial_cost_ms and, for a full
+// search, records it in the system's tune cache for this pulse count.
 static int tune_gemm(mpci_prop *pr, const GemmCand *only) {
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   int rc = set_all_live(pr, 0);
   float best = 1e30f;
+  double best_cost = 1e30;
   GemmCand bc = only ? *only : kGemmCands[0];
   if (rc == MPCI_OK) {
     if (only) {
@@ -646,7 +660,9 @@ static int tune_gemm(mpci_prop *pr, const GemmCand *only) {
         float ms = 0.f;
         rc = time_gemm_cand(pr, cd, e0, e1, &ms);
         if (rc != MPCI_OK) break;
-        if (ms < best) {
+        const double cost = ms + partial_cost_ms(pr, cd.ks);
+        if (cost < best_cost) {
+          best_cost = cost;
           best = ms;
           bc = cd;
         }
@@ -695,7 +711,7 @@ extern "C" int mpci_prop_create(mpci_system *sys, int P, mpci_prop **out) {
   pr->NP = (long long)sys->N * P;
   const int rpb = 256 / P;
   pr->nblk = (sys->N + rpb - 1) / rpb;
-  const int rps = P == 1 ? 64 : rpb;  // k_stage<4> for one pulse
+  const int rps = P == 1 ? 256 / kStageSub1 : rpb;  // rows per k_stage block
   pr->nblk_stage = (sys->N + rps - 1) / rps;
   TRY(init_desc(pr.get(), 1));
   TRY(pr->X.alloc(pr->NP));
@@ -827,8 +843,8 @@ static int enqueue_tile(mpci_prop *pr, const double2 *xin, int s) {
 
 static int launch_stage(mpci_prop *pr, const StageArgs &a) {
   if (pr->P == 1)
-    hipLaunchKernelGGL(k_stage<4>, dim3(pr->nblk_stage), dim3(256), 0,
-                       pr->stream, a, pr->desc);
+    hipLaunchKernelGGL(k_stage<kStageSub1>, dim3(pr->nblk_stage), dim3(256),
+                       0, pr->stream, a, pr->desc);
   else
     hipLaunchKernelGGL(k_stage<1>, dim3(pr->nblk_stage), dim3(256), 0,
                        pr->stream, a, pr->desc);

@@ -985,12 +985,15 @@ __device__ __forceinline__ void sum_both(const double2 *cb, int nc,
 }
 
 // Stage kernel.  SUB lanes share one (row, pulse): they split the partial
-// sums (every SUB-th partial each) and combine them with a DPP quad
-// reduction, lane 0 of the group doing the rest.  SUB = 4 for one pulse
-// (dozens of partials per row), 1 for the batched form (K-split partials).
+// sums (every SUB-th partial each) and combine them with a DPP reduction
+// inside the group (quad_perm steps, then row_half_mirror and row_mirror
+// for 8 and 16 lanes), lane 0 of the group doing the rest.  SUB > 1 for
+// one pulse (dozens of partials per row; 4 measured fastest on the He
+// workload, against 8 and 16), 1 for the batched form (K-split partials).
 template <int SUB>
 __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
-  static_assert(SUB == 1 || SUB == 4, "lanes per row: 1 or a DPP quad");
+  static_assert(SUB == 1 || SUB == 4 || SUB == 8 || SUB == 16,
+                "lanes per row: 1, 4, 8 or 16");
   const int P = d.P;
   const int t = threadIdx.x;
   const int sub = t & (SUB - 1);
@@ -1016,7 +1019,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
                     a.part + (L < d.nb - 1 ? d.row_part_off[L] : 0),
                     L < d.nb - 1 ? d.nrow_part[L] : 0, (long long)sL * P,
                     (long long)i * P + p, sub, cs, rs);
-      if (SUB == 4) {
+      if (SUB >= 4) {
         cs.x += dpp_d<0xB1>(cs.x);
         cs.y += dpp_d<0xB1>(cs.y);
         rs.x += dpp_d<0xB1>(rs.x);
@@ -1025,6 +1028,18 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
         cs.y += dpp_d<0x4E>(cs.y);
         rs.x += dpp_d<0x4E>(rs.x);
         rs.y += dpp_d<0x4E>(rs.y);
+      }
+      if (SUB >= 8) {
+        cs.x += dpp_d<0x141>(cs.x);
+        cs.y += dpp_d<0x141>(cs.y);
+        rs.x += dpp_d<0x141>(rs.x);
+        rs.y += dpp_d<0x141>(rs.y);
+      }
+      if (SUB >= 16) {
+        cs.x += dpp_d<0x140>(cs.x);
+        cs.y += dpp_d<0x140>(cs.y);
+        rs.x += dpp_d<0x140>(rs.x);
+        rs.y += dpp_d<0x140>(rs.y);
       }
       if (sub == 0) {
         const double2 xs = (s == 0) ? a.X[idx] : a.Xs[idx];
@@ -1124,7 +1139,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
   }
 }
 template __global__ void k_stage<1>(StageArgs, SysDesc);
-template __global__ void k_stage<4>(StageArgs, SysDesc);
+template __global__ void k_stage<kStageSub1>(StageArgs, SysDesc);
 
 // ---------------------------------------------------------------------
 // Per-pulse reduction and step control; one workgroup per pulse.
