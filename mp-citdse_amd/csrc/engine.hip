@@ -842,12 +842,22 @@ static int enqueue_tile(mpci_prop *pr, const double2 *xin, int s) {
 }
 
 static int launch_stage(mpci_prop *pr, const StageArgs &a) {
-  if (pr->P == 1)
-    hipLaunchKernelGGL(k_stage<kStageSub1>, dim3(pr->nblk_stage), dim3(256),
-                       0, pr->stream, a, pr->desc);
-  else
-    hipLaunchKernelGGL(k_stage<1>, dim3(pr->nblk_stage), dim3(256), 0,
-                       pr->stream, a, pr->desc);
+  const dim3 grid(pr->nblk_stage), block(256);
+  const bool last = a.s == kStages - 1;
+  if (pr->P == 1) {
+    if (last)
+      hipLaunchKernelGGL((k_stage<kStageSub1, true>), grid, block, 0,
+                         pr->stream, a, pr->desc);
+    else
+      hipLaunchKernelGGL((k_stage<kStageSub1, false>), grid, block, 0,
+                         pr->stream, a, pr->desc);
+  } else if (last) {
+    hipLaunchKernelGGL((k_stage<1, true>), grid, block, 0, pr->stream, a,
+                       pr->desc);
+  } else {
+    hipLaunchKernelGGL((k_stage<1, false>), grid, block, 0, pr->stream, a,
+                       pr->desc);
+  }
   HIPCHK(hipGetLastError());
   return MPCI_OK;
 }

@@ -958,7 +958,9 @@ __device__ __forceinline__ double2 sum_partials(const double2 *base, int n,
 
 // Partials q = sub, sub + SUB, ... of both sums (column partials of D_{L-1}^T
 // x_{L-1}, row partials of D_L x_{L+1}), up to 8 of each in flight at once.
-template <int SUB>
+// U partials per lane in flight at once; the sums run in partial order
+// whatever U is.
+template <int SUB, int U>
 __device__ __forceinline__ void sum_both(const double2 *cb, int nc,
                                          const double2 *rb, int nr,
                                          long long stride, long long at,
@@ -966,16 +968,16 @@ __device__ __forceinline__ void sum_both(const double2 *cb, int nc,
   cs = make_double2(0.0, 0.0);
   rs = make_double2(0.0, 0.0);
   const int n = nc > nr ? nc : nr;
-  for (int q0 = sub; q0 < n; q0 += 8 * SUB) {
-    double2 vc[8], vr[8];
+  for (int q0 = sub; q0 < n; q0 += U * SUB) {
+    double2 vc[U], vr[U];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int q = q0 + u * SUB;
       vc[u] = q < nc ? cb[(long long)q * stride + at] : make_double2(0.0, 0.0);
       vr[u] = q < nr ? rb[(long long)q * stride + at] : make_double2(0.0, 0.0);
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < U; ++u) {
       cs.x += vc[u].x;
       cs.y += vc[u].y;
       rs.x += vr[u].x;
@@ -990,7 +992,9 @@ __device__ __forceinline__ void sum_both(const double2 *cb, int nc,
 // for 8 and 16 lanes), lane 0 of the group doing the rest.  SUB > 1 for
 // one pulse (dozens of partials per row; 4 measured fastest on the He
 // workload, against 8 and 16), 1 for the batched form (K-split partials).
-template <int SUB>
+// LAST: the thirteenth stage (solution, error and norm partials), a
+// separate instantiation so the other twelve carry its registers.
+template <int SUB, bool LAST>
 __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
   static_assert(SUB == 1 || SUB == 4 || SUB == 8 || SUB == 16,
                 "lanes per row: 1, 4, 8 or 16");
@@ -1014,11 +1018,12 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
       const int sL = d.off[L + 1] - d.off[L];
       const long long idx = (long long)row * P + p;
       double2 cs, rs;
-      sum_both<SUB>(a.part + (L > 0 ? d.col_part_off[L - 1] : 0),
-                    L > 0 ? d.ncol_part[L - 1] : 0,
-                    a.part + (L < d.nb - 1 ? d.row_part_off[L] : 0),
-                    L < d.nb - 1 ? d.nrow_part[L] : 0, (long long)sL * P,
-                    (long long)i * P + p, sub, cs, rs);
+      sum_both<SUB, SUB == 1 ? 4 : 8>(
+          a.part + (L > 0 ? d.col_part_off[L - 1] : 0),
+          L > 0 ? d.ncol_part[L - 1] : 0,
+          a.part + (L < d.nb - 1 ? d.row_part_off[L] : 0),
+          L < d.nb - 1 ? d.nrow_part[L] : 0, (long long)sL * P,
+          (long long)i * P + p, sub, cs, rs);
       if (SUB >= 4) {
         cs.x += dpp_d<0xB1>(cs.x);
         cs.y += dpp_d<0xB1>(cs.y);
@@ -1065,7 +1070,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
         if (c.mode != kModeRhs) {
           const double h = c.hc;
           const double2 x = a.X[idx];
-          if (s < kStages - 1) {
+          if (!LAST) {
             double vx = 1.0 * x.x, vy = 1.0 * x.y;
             // the tableau's zero entries (22 of the 66 below the diagonal
             // before the last row) skip their loads: a uniform branch on a
@@ -1121,7 +1126,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
       }  // sub == 0
     }
   }
-  if (s == kStages - 1 && a.red != nullptr) {
+  if (LAST && a.red != nullptr) {
     __shared__ double se[256], sn[256];
     se[t] = errv;
     sn[t] = nrm2;
@@ -1138,8 +1143,10 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
     }
   }
 }
-template __global__ void k_stage<1>(StageArgs, SysDesc);
-template __global__ void k_stage<kStageSub1>(StageArgs, SysDesc);
+template __global__ void k_stage<1, false>(StageArgs, SysDesc);
+template __global__ void k_stage<1, true>(StageArgs, SysDesc);
+template __global__ void k_stage<kStageSub1, false>(StageArgs, SysDesc);
+template __global__ void k_stage<kStageSub1, true>(StageArgs, SysDesc);
 
 // ---------------------------------------------------------------------
 // Per-pulse reduction and step control; one workgroup per pulse.

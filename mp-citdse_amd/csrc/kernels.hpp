@@ -63,7 +63,7 @@ template <int WM, int WN>
 __global__ void k_gemm_coupling(const GemmJob *jobs, const double *Dr,
                                 const long long *dr_off, const double *X,
                                 double *part, SysDesc d, const Ctl *ctl);
-template <int SUB>
+template <int SUB, bool LAST>
 __global__ void k_stage(StageArgs a, SysDesc d);
 __global__ void k_finalize(Ctl *ctl, const double *red, int nblk, int P,
                            const double *hs);
