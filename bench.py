@@ -98,6 +98,43 @@ def run_gpu(args, mpci, synth, sysg):
             "tau": tau}
 
 
+S_HE_TEST = [3030, 2799, 2808, 2808]  # inp/He_test.yaml states_in_l
+
+
+def run_he_test_shape(mpci, synth, attempts=300):
+    """The block sizes of the reference's own He input (uneven, ragged
+    against the GEMV units), synthetic E and D of that shape, the same
+    single pulse and tolerance as the headline: graph-replayed attempts,
+    then the same attempts instrumented for the coupling kernel's time."""
+    E, D = synth.system_arrays(S_HE_TEST, seed=synth.HE_SEED + 7)
+    sysg = mpci.System(E, D, gauge="v")
+    prop = make_prop(mpci, sysg)
+    prop.set_pulse(mpci.Pulse.sine(E0_AU ** 2, W_AU, CYCLES, gauge="v"))
+    tau = mpci.period(W_AU, CYCLES)
+    x0 = synth.ground_state(sysg.N)
+    prop.set_state(x0)
+    prop.prop_adaptive(0.0, tau, 0.05, TOL, TOL, max_attempts=10,
+                       allow_limit=True)
+    prop.set_state(x0)
+    t0 = time.perf_counter()
+    st, _ = prop.prop_adaptive(0.0, tau, 0.05, TOL, TOL,
+                               max_attempts=attempts, allow_limit=True)
+    wall = time.perf_counter() - t0
+    prop.set_state(x0)
+    prop.set_instrument(True)
+    prop.prop_adaptive(0.0, tau, 0.05, TOL, TOL, max_attempts=attempts,
+                       allow_limit=True)
+    kms, kl, kbytes, _ = prop.kernel_time()
+    prop.set_instrument(False)
+    avg_us = kms / max(kl, 1) * 1e3
+    gbs = kbytes / (avg_us * 1e-6) / 1e9
+    return {"states_in_l": S_HE_TEST, "N": sysg.N,
+            "dipole_elements": sysg.dipole_elements,
+            "accepted": st[0].accepted, "steps_per_s": st[0].accepted / wall,
+            "k_gemv_tile_avg_us": avg_us, "achieved_gbs": gbs,
+            "frac_of_8tbs": gbs / HBM_PEAK_GBS}
+
+
 def run_reference_mode(mpci, synth, sysg, steps=200, dt=0.1):
     """Fixed dt, field frozen at t + dt per step (src/tdse/tdse.cpp:187)."""
     prop = make_prop(mpci, sysg)
@@ -465,6 +502,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--no-reference-mode", action="store_true")
+    ap.add_argument("--no-he-test-shape", action="store_true",
+                    help="skip the leg on the block sizes of the "
+                         "reference's He input")
     ap.add_argument("--no-w2e", action="store_true",
                     help="skip the w2e and d1e producer legs")
     ap.add_argument("--no-graph", action="store_true",
@@ -529,6 +569,11 @@ def main():
     if ref is not None:
         out["reference_mode"] = {"steps_per_s": round(ref["steps_per_s"], 3),
                                  "dt": ref["dt"], "steps": ref["steps"]}
+    if not args.no_he_test_shape:
+        log("bench: He_test.yaml block sizes (3030, 2799, 2808, 2808) ...")
+        h = run_he_test_shape(mpci, synth)
+        out["he_test_shape"] = {k: (round(v, 4) if isinstance(v, float)
+                                    else v) for k, v in h.items()}
     if not args.no_batch:
         log("bench: batch-64 ...")
         b = run_batch(mpci, synth, sysg)

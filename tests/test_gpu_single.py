@@ -201,3 +201,31 @@ def test_full_he_shape_two_steps(mpci, oracle):
         xr /= math.sqrt(np.dot(xr, xr))
     assert np.abs(x - xr).max() < 1e-12
     assert abs(np.dot(x, x) - 1.0) < 1e-13
+
+
+@pytest.mark.parametrize("gauge", ["v", "l"])
+def test_reference_he_test_shape_two_steps(mpci, oracle, gauge):
+    """The block sizes of the reference's own He input
+    (inp/He_test.yaml: states_in_l [3030, 2799, 2808, 2808]), uneven and
+    ragged against every GEMV unit: two reference-mode steps in both
+    gauges, and the batched form (P = 2) on the same system."""
+    s = [3030, 2799, 2808, 2808]
+    g, o = build(mpci, oracle, s, gauge, seed=31)
+    pulse = oracle.Pulse.sine(0.03 ** 2, 0.55, 12)
+    F = fixed_fields(pulse, 5.0, 0.1, 2)
+    x0 = np.random.default_rng(7).standard_normal(2 * g.N)
+    x0 /= np.linalg.norm(x0)
+    xr = x0.copy()
+    for k in range(2):
+        xr = o.step(F[0, k], xr, 0.1)
+        xr /= math.sqrt(np.dot(xr, xr))
+    prop = mpci.Propagator(g)
+    prop.set_state(x0)
+    prop.prop_fixed(5.0, 0.1, F)
+    assert np.abs(prop.get_state() - xr).max() < 1e-12
+    batch = mpci.Propagator(g, P=2)
+    for p in range(2):
+        batch.set_state(x0, p)
+    batch.prop_fixed(5.0, 0.1, np.vstack([F, F]))
+    for p in range(2):
+        assert np.abs(batch.get_state(p) - xr).max() < 1e-12, p
