@@ -1,7 +1,7 @@
 // Stand-alone driver of the single-pulse hot path over the C ABI, for
 // hardware-counter runs that need a plain process (no Python runtime).
 //
-//   tdse_he_bench [attempts] [copy_mib]
+//   tdse_he_bench [attempts] [copy_mib] [batch_pulses]
 //
 // Builds the seeded synthetic He system (4 blocks of 4000 states, velocity
 // gauge), propagates one sin^2 pulse (w = 0.7, E0 = 0.03, 10 cycles) from the
@@ -10,6 +10,11 @@
 // device to device three times.  The copy reads and writes exactly
 // copy_mib MiB per launch, which scales byte counters (FETCH_SIZE,
 // WRITE_SIZE) measured for the coupling kernel in the same run.
+//
+// With batch_pulses = P > 1 it then propagates P pulses on the same system
+// in the batched form (FP64 MFMA coupling, autotuned tile) for the same
+// number of attempts: w ~ U[0.5, 1], E0 log-uniform in [0.005, 0.05],
+// 4-20 cycles, drawn from the seeded generator as bench.py draws them.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -32,6 +37,7 @@ static int check(int rc, const char *what) {
 int main(int argc, char **argv) {
   const int attempts = argc > 1 ? std::atoi(argv[1]) : 20;
   const long long copy_mib = argc > 2 ? std::atoll(argv[2]) : 1024;
+  const int batch = argc > 3 ? std::atoi(argv[3]) : 0;
   const int nb = 4;
   const int s[nb] = {4000, 4000, 4000, 4000};
   std::vector<std::vector<double>> E(nb), D(nb - 1);
@@ -88,6 +94,44 @@ int main(int argc, char **argv) {
     (void)hipFree(dst);
   }
   mpci_prop_destroy(prop);
+  if (batch > 1) {
+    const int P = batch;
+    mpci_prop *bp = nullptr;
+    check(mpci_prop_create(sys, P, &bp), "mpci_prop_create (batch)");
+    check(mpci_prop_set_graph(bp, 0), "mpci_prop_set_graph");
+    // the pulse batch of mpci/synth.py pulse_batch (seed + 3, stream 1000)
+    std::vector<double> u(3 * (size_t)P);
+    mpci_synth_uniform(20261202ULL + 3, 1000, 3LL * P, 0.0, 1.0, u.data());
+    std::vector<double> bt0(P, 0.0), bt1(P), bh0(P, 0.05);
+    for (int p = 0; p < P; ++p) {
+      const double wp = 0.5 + 0.5 * u[p];
+      const double ep = std::exp(std::log(0.005) +
+                                 (std::log(0.05) - std::log(0.005)) * u[P + p]);
+      int cyc = (int)(u[2 * P + p] * 17);
+      cyc = 4 + (cyc < 16 ? cyc : 16);
+      mpci_pulse pp;
+      check(mpci_pulse_sine(ep * ep, wp, 0.0, cyc, 'v', &pp),
+            "mpci_pulse_sine");
+      check(mpci_prop_set_pulse(bp, p, &pp), "mpci_prop_set_pulse");
+      check(mpci_prop_set_state(bp, p, x.data()), "mpci_prop_set_state");
+      bt1[p] = mpci_period(wp, cyc);
+    }
+    std::vector<mpci_stats> bst(P);
+    a = std::chrono::steady_clock::now();
+    const int brc = mpci_prop_adaptive(bp, bt0.data(), bt1.data(), bh0.data(),
+                                       1e-10, 1e-10, attempts, 0, bst.data(),
+                                       nullptr, 0);
+    if (brc != MPCI_OK && brc != MPCI_ERR_LIMIT)
+      check(brc, "mpci_prop_adaptive (batch)");
+    const double bsec =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - a)
+            .count();
+    long long acc = 0;
+    for (const auto &b : bst) acc += b.accepted;
+    std::printf("batch P=%d: %lld accepted pulse-steps in %.3f s\n", P, acc,
+                bsec);
+    mpci_prop_destroy(bp);
+  }
   mpci_system_destroy(sys);
   return 0;
 }
