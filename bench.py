@@ -278,15 +278,23 @@ def run_c1(mpci, synth, c1):
     for k in range(steps):
         F[0, k] = pulse(t + dt)
         t += dt
-    prop = make_prop(mpci, g)
     x0 = synth.ground_state(g.N)
-    prop.set_state(x0)
-    prop.prop_fixed(0.0, dt, F[:, :20])
-    prop.set_state(x0)
-    t0 = time.perf_counter()
-    prop.prop_fixed(0.0, dt, F)
-    wall = time.perf_counter() - t0
-    return {"steps": steps, "steps_per_s": round(steps / wall, 1),
+    rates = {}
+    for mode in (-1, 0):  # the default (fused small-system path), the tiles
+        prop = make_prop(mpci, g)
+        on = prop.set_small_path(mode)
+        prop.set_state(x0)
+        prop.prop_fixed(0.0, dt, F[:, :20])
+        prop.set_state(x0)
+        t0 = time.perf_counter()
+        prop.prop_fixed(0.0, dt, F)
+        rates["small_path" if on else "tile_path"] = \
+            steps / (time.perf_counter() - t0)
+        prop.close()
+    return {"steps": steps,
+            "steps_per_s": round(rates["small_path"], 1),
+            "tile_path_steps_per_s": round(rates["tile_path"], 1),
+            "path": "fused small-system stage kernel (one launch per stage)",
             "bound": "launch/latency: 0.94 MB of dipoles stay in cache, so "
                      "no HBM roofline applies"}
 

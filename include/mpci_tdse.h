@@ -187,6 +187,15 @@ int mpci_prop_populations(mpci_prop *prop, int nobs, const double *W,
 /* Launch the 13-stage step as one captured hipGraph (1, default) or as
  * individual launches (0). */
 int mpci_prop_set_graph(mpci_prop *prop, int enable);
+/* One pulse on a small system: each stage as one fused launch (coupling
+ * from row-major copies of D_L and D_L^T, one wave per row, then the stage
+ * algebra) instead of the GEMV tile kernel and the stage kernel.  mode -1
+ * (default): on when P = 1, the system has more than one block and its
+ * dipoles take at most 8 MiB; 0: off; 1: on whenever P = 1 and there is
+ * more than one block.  active (may be NULL) reports the outcome.  Both
+ * paths give the same result to rounding (the coupling sums in another
+ * order). */
+int mpci_prop_set_small_path(mpci_prop *prop, int mode, int *active);
 /* Instrumentation: when on, steps are launched individually and every
  * coupling kernel carries a pair of HIP events on its own dispatch
  * (hipExtLaunchKernelGGL), so each interval is that kernel's execution;

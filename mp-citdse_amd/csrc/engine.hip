@@ -1,7 +1,9 @@
 // Host side of the MI355X tdse engine and its C ABI (include/mpci_tdse.h).
 //
-// One RKF78 attempt = 13 x (k_gemv_tile, k_stage) + k_finalize + k_apply,
-// captured once into a hipGraph and replayed.  Step control lives in
+// One RKF78 attempt = 13 x (coupling, k_stage) + k_finalize + k_apply, the
+// coupling being k_gemv_tile for one pulse and an MFMA GEMM for a batch;
+// one pulse on a small system takes 13 fused k_small_stage launches
+// instead.  The attempt is captured once into a hipGraph and replayed.  Step control lives in
 // device memory (Ctl), so the host only re-launches the graph in chunks and
 // drains the per-step log between chunks.
 #include <hip/hip_ext.h>
@@ -90,6 +92,12 @@ struct mpci_system {
   DevBuf<double> Dr;
   DevBuf<long long> dr_off;
   std::vector<int> dr_ld;  // even row stride of each block in Dr
+  DevBuf<int> dr_ld_d;     // the same on the device
+  // row-major D_L^T (s_{L+1} x s_L) of every pair, for the single-pulse
+  // small-system path, built with it
+  DevBuf<double> DrT;
+  DevBuf<long long> drt_off;
+  DevBuf<int> drt_ld_d;
   std::vector<std::vector<double>> hostD;
   // autotuned GEMM tile per batch size P: every propagator of this system
   // with the same P uses the same tiles, hence the same summation order
@@ -108,7 +116,12 @@ struct mpci_prop {
   long long NP = 0;
   int nblk = 0;           // workgroups of the per-row kernels (256 rows / P)
   int nblk_stage = 0;     // k_stage workgroups (64 rows for one pulse)
+  // one pulse on a small system: one fused launch per stage
+  // (k_small_stage, four rows per workgroup) instead of GEMV tile + stage
+  bool small = false;
+  int nblk_small = 0;
   DevBuf<double2> X, Xs, K, Y, part;
+  DevBuf<double2> Xs2;  // second stage-input buffer of the fused path
   DevBuf<double> red;
   DevBuf<Ctl> ctl;
   DevBuf<PulseDev> pulses;
@@ -417,6 +430,9 @@ static int ensure_rowmajor(mpci_system *sys) {
   }
   TRY(sys->Dr.alloc(tot));
   TRY(sys->dr_off.alloc(off.size()));
+  TRY(sys->dr_ld_d.alloc(off.size()));
+  HIPCHK(hipMemcpy(sys->dr_ld_d.p, sys->dr_ld.data(),
+                   sizeof(int) * off.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(sys->Dr.p, 0, sizeof(double) * tot));
   HIPCHK(hipMemcpy(sys->dr_off.p, off.data(), sizeof(long long) * off.size(),
                    hipMemcpyHostToDevice));
@@ -426,6 +442,37 @@ static int ensure_rowmajor(mpci_system *sys) {
                        sizeof(double) * sys->s[L + 1], sys->s[L],
                        hipMemcpyHostToDevice));
   HIPCHK(hipDeviceSynchronize());
+  return MPCI_OK;
+}
+
+// Row-major D_L^T of every pair (row stride s_L) for k_small_stage.
+static int ensure_transposed(mpci_system *sys) {
+  if (sys->DrT.p || sys->nb < 2) return MPCI_OK;
+  std::vector<long long> off(sys->nb - 1);
+  std::vector<int> ld(sys->nb - 1);
+  long long tot = 0;
+  for (int L = 0; L + 1 < sys->nb; ++L) {
+    ld[L] = sys->s[L];
+    off[L] = tot;
+    tot += (long long)sys->s[L + 1] * sys->s[L];
+  }
+  std::vector<double> h((size_t)tot);
+  for (int L = 0; L + 1 < sys->nb; ++L) {
+    const int a = sys->s[L], b = sys->s[L + 1];
+    const double *D = sys->hostD[L].data();
+    double *T = h.data() + off[L];
+    for (int i = 0; i < a; ++i)
+      for (int j = 0; j < b; ++j) T[(size_t)j * a + i] = D[(size_t)i * b + j];
+  }
+  TRY(sys->DrT.alloc(tot));
+  TRY(sys->drt_off.alloc(off.size()));
+  TRY(sys->drt_ld_d.alloc(ld.size()));
+  HIPCHK(hipMemcpy(sys->DrT.p, h.data(), sizeof(double) * tot,
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(sys->drt_off.p, off.data(), sizeof(long long) * off.size(),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(sys->drt_ld_d.p, ld.data(), sizeof(int) * ld.size(),
+                   hipMemcpyHostToDevice));
   return MPCI_OK;
 }
 
@@ -697,6 +744,28 @@ static int autotune_gemm(mpci_prop *pr) {
   return tune_gemm(pr, nullptr);
 }
 
+// Dipole bytes (8 S) up to which one pulse takes the small-system path by
+// default: there every stage is a handful of microseconds of launch and
+// latency, and the fused kernel halves the launches per attempt.
+constexpr long long kSmallPathBytes = 8LL << 20;
+
+// mode -1: automatic (one pulse, more than one block, 8 S within
+// kSmallPathBytes); 0: off; 1: on wherever it applies (one pulse, more
+// than one block).
+static int apply_small_path(mpci_prop *pr, int mode) {
+  mpci_system *sys = pr->sys;
+  bool on = false;
+  if (pr->P == 1 && sys->nb > 1)
+    on = mode == 1 || (mode < 0 && 8 * sys->S <= kSmallPathBytes);
+  if (on) {
+    TRY(ensure_rowmajor(sys));
+    TRY(ensure_transposed(sys));
+    if (!pr->Xs2.p) TRY(pr->Xs2.alloc(pr->NP));
+  }
+  pr->small = on;
+  return MPCI_OK;
+}
+
 extern "C" int mpci_prop_create(mpci_system *sys, int P, mpci_prop **out) {
   if (!out || !sys) return fail(MPCI_ERR_ARG, "NULL argument");
   *out = nullptr;
@@ -718,7 +787,9 @@ extern "C" int mpci_prop_create(mpci_system *sys, int P, mpci_prop **out) {
   TRY(pr->Xs.alloc(pr->NP));
   TRY(pr->Y.alloc(pr->NP));
   TRY(pr->K.alloc(pr->NP * kStages));
-  TRY(pr->red.alloc((size_t)std::max(pr->nblk, pr->nblk_stage) * P * 2));
+  pr->nblk_small = (sys->N + 3) / 4;
+  TRY(pr->red.alloc(
+      (size_t)std::max({pr->nblk, pr->nblk_stage, pr->nblk_small}) * P * 2));
   TRY(pr->ctl.alloc(P));
   TRY(pr->pulses.alloc(P));
   TRY(pr->log.alloc((size_t)pr->logcap * P));
@@ -736,6 +807,7 @@ extern "C" int mpci_prop_create(mpci_system *sys, int P, mpci_prop **out) {
     TRY(ensure_rowmajor(sys));
     TRY(autotune_gemm(pr.get()));
   }
+  TRY(apply_small_path(pr.get(), -1));
   *out = pr.release();
   return MPCI_OK;
 }
@@ -866,6 +938,14 @@ static StageArgs stage_args(mpci_prop *pr, int s) {
   StageArgs a{};
   a.X = pr->X.p;
   a.Xs = pr->Xs.p;
+  a.Xs_next = pr->Xs.p;
+  if (pr->small) {
+    // stage s reads its input from one buffer while it writes stage s + 1's
+    // into the other (stage 0 reads X)
+    double2 *buf[2] = {pr->Xs.p, pr->Xs2.p};
+    a.Xs = buf[s & 1];
+    a.Xs_next = buf[(s + 1) & 1];
+  }
   a.K = pr->K.p;
   a.Y = pr->Y.p;
   a.part = pr->part.p;
@@ -881,14 +961,44 @@ static StageArgs stage_args(mpci_prop *pr, int s) {
   return a;
 }
 
+// Stage s on the small-system path (the instrumented event pair rides on
+// the fused kernel's dispatch).
+static int launch_small_stage(mpci_prop *pr, const StageArgs &a, int s) {
+  mpci_system *sys = pr->sys;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (pr->instrument) {
+    e0 = pr->ev[pr->ev_base + 2 * s];
+    e1 = pr->ev[pr->ev_base + 2 * s + 1];
+  }
+  const dim3 grid(pr->nblk_small), block(256);
+  if (a.s == kStages - 1)
+    hipExtLaunchKernelGGL((k_small_stage<true>), grid, block, 0, pr->stream,
+                          e0, e1, 0, a, pr->desc, sys->Dr.p, sys->dr_off.p,
+                          sys->dr_ld_d.p, sys->DrT.p, sys->drt_off.p,
+                          sys->drt_ld_d.p);
+  else
+    hipExtLaunchKernelGGL((k_small_stage<false>), grid, block, 0, pr->stream,
+                          e0, e1, 0, a, pr->desc, sys->Dr.p, sys->dr_off.p,
+                          sys->dr_ld_d.p, sys->DrT.p, sys->drt_off.p,
+                          sys->drt_ld_d.p);
+  HIPCHK(hipGetLastError());
+  return MPCI_OK;
+}
+
 static int enqueue_attempt(mpci_prop *pr) {
   for (int s = 0; s < kStages; ++s) {
-    TRY(enqueue_tile(pr, s == 0 ? pr->X.p : pr->Xs.p, s));
-    TRY(launch_stage(pr, stage_args(pr, s)));
+    if (pr->small) {
+      TRY(launch_small_stage(pr, stage_args(pr, s), s));
+    } else {
+      TRY(enqueue_tile(pr, s == 0 ? pr->X.p : pr->Xs.p, s));
+      TRY(launch_stage(pr, stage_args(pr, s)));
+    }
     HIPCHK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_finalize, dim3(pr->P), dim3(256), 0, pr->stream,
-                     pr->ctl.p, pr->red.p, pr->nblk_stage, pr->P, pr->hs.p);
+                     pr->ctl.p, pr->red.p,
+                     pr->small ? pr->nblk_small : pr->nblk_stage, pr->P,
+                     pr->hs.p);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_apply, dim3(pr->nblk), dim3(256), 0, pr->stream,
                      pr->X.p, pr->Y.p, pr->ctl.p, pr->log.p, pr->logcap,
@@ -1026,10 +1136,14 @@ extern "C" int mpci_prop_rhs(mpci_prop *pr, const double *F, const double *x,
     c[p].Fc = F[p];
   }
   TRY(write_ctl(pr, c));
-  TRY(enqueue_tile(pr, pr->X.p, 0));
   StageArgs a = stage_args(pr, 0);
   a.red = nullptr;
-  TRY(launch_stage(pr, a));
+  if (pr->small) {
+    TRY(launch_small_stage(pr, a, 0));
+  } else {
+    TRY(enqueue_tile(pr, pr->X.p, 0));
+    TRY(launch_stage(pr, a));
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(pr->stream));
   std::vector<double2> k(pr->NP);
@@ -1644,4 +1758,16 @@ extern "C" int mpci_prop_set_gemm_candidate(mpci_prop *pr, int index) {
   HIPCHK(hipStreamSynchronize(pr->stream));
   drop_graph(pr);
   return tune_gemm(pr, &kGemmCands[index]);
+}
+
+extern "C" int mpci_prop_set_small_path(mpci_prop *pr, int mode,
+                                        int *active) {
+  if (!pr || mode < -1 || mode > 1)
+    return fail(MPCI_ERR_ARG, "mpci_prop_set_small_path: bad argument");
+  HIPCHK(hipSetDevice(pr->sys->device));
+  HIPCHK(hipStreamSynchronize(pr->stream));
+  drop_graph(pr);  // the captured step holds the other path's kernels
+  TRY(apply_small_path(pr, mode));
+  if (active) *active = pr->small ? 1 : 0;
+  return MPCI_OK;
 }

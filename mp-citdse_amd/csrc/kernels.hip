@@ -986,6 +986,96 @@ __device__ __forceinline__ void sum_both(const double2 *cb, int nc,
   }
 }
 
+// The per-row end of a stage, shared by k_stage and k_small_stage: from
+// the coupling sums cs = D^T x_{L-1} and rs = D x_{L+1} of (row, pulse p)
+// it forms k_s, stores it, and forms the next stage input, or on the
+// thirteenth stage the solution, the embedded error ratio (errv) and
+// |y|^2 (nrm2).
+template <bool LAST>
+__device__ __forceinline__ void stage_row(const StageArgs &a,
+                                          const SysDesc &d, const Ctl &c,
+                                          int row, int p, long long idx,
+                                          double2 cs, double2 rs,
+                                          double &errv, double &nrm2) {
+  const int s = a.s;
+  const double2 xs = (s == 0) ? a.X[idx] : a.Xs[idx];
+  const double e = a.E[row];
+  double F;
+  if (c.fmode == kFieldTable)
+    F = a.Ftab[(long long)p * c.hstride + c.step];
+  else if (c.fmode == kFieldPulse)
+    F = field_at(a.pulses[p], a.env, c.t + cC[s] * c.hc);
+  else
+    F = c.Fc;
+  double2 k;
+  if (d.gauge == 0) {  // velocity: -iE x + F (D^T x_{L-1} - D x_{L+1})
+    k.x = e * xs.y + F * (cs.x - rs.x);
+    k.y = -e * xs.x + F * (cs.y - rs.y);
+  } else {             // length: -i (E x + F (D^T x_{L-1} + D x_{L+1}))
+    const double cx = cs.x + rs.x, cy = cs.y + rs.y;
+    k.x = e * xs.y + F * cy;
+    k.y = -e * xs.x - F * cx;
+  }
+  const long long NP = a.NP;
+  a.K[s * NP + idx] = k;
+  if (c.mode != kModeRhs) {
+    const double h = c.hc;
+    const double2 x = a.X[idx];
+    if (!LAST) {
+      double vx = 1.0 * x.x, vy = 1.0 * x.y;
+      // the tableau's zero entries (22 of the 66 below the diagonal
+      // before the last row) skip their loads: a uniform branch on a
+      // constant-memory value
+      double2 kj[kStages - 1];
+#pragma unroll
+      for (int j = 0; j < kStages - 1; ++j)
+        kj[j] = (j < s && cA[s + 1][j] != 0.0) ? a.K[j * NP + idx]
+                                               : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < kStages - 1; ++j) {
+        if (j < s) {
+          const double co = h * cA[s + 1][j];
+          vx += co * kj[j].x;
+          vy += co * kj[j].y;
+        }
+      }
+      const double co = h * cA[s + 1][s];
+      vx += co * k.x;
+      vy += co * k.y;
+      a.Xs_next[idx] = make_double2(vx, vy);
+      if (a.xs_store)
+        a.xs_store[((long long)c.step * (kStages - 1) + s) * NP + idx] =
+            make_double2(vx, vy);
+    } else {
+      double yx = 1.0 * x.x, yy = 1.0 * x.y, ex = 0.0, ey = 0.0;
+      // k_2 .. k_5 carry zero weight in both b and b - bhat
+      double2 kj[kStages];
+#pragma unroll
+      for (int j = 0; j < kStages - 1; ++j)
+        kj[j] = (j == 0 || cB[j] != 0.0 || cDB[j] != 0.0)
+                    ? a.K[j * NP + idx]
+                    : make_double2(0.0, 0.0);
+      kj[kStages - 1] = k;
+      const double2 k1 = kj[0];
+#pragma unroll
+      for (int j = 0; j < kStages; ++j) {
+        const double cb = h * cB[j], cd = h * cDB[j];
+        yx += cb * kj[j].x;
+        yy += cb * kj[j].y;
+        ex += cd * kj[j].x;
+        ey += cd * kj[j].y;
+      }
+      a.Y[idx] = make_double2(yx, yy);
+      const double rx =
+          fabs(ex) / (c.atol + c.rtol * (fabs(x.x) + h * fabs(k1.x)));
+      const double ry =
+          fabs(ey) / (c.atol + c.rtol * (fabs(x.y) + h * fabs(k1.y)));
+      errv = rx > ry ? rx : ry;
+      nrm2 = yx * yx + yy * yy;
+    }
+  }
+}
+
 // Stage kernel.  SUB lanes share one (row, pulse): they split the partial
 // sums (every SUB-th partial each) and combine them with a DPP reduction
 // inside the group (quad_perm steps, then row_half_mirror and row_mirror
@@ -1006,7 +1096,6 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
   const int lr = tt / P, p = tt - lr * P;
   const int row = blockIdx.x * rpb + lr;
   const bool active_lane = (lr < rpb) && (row < d.N);
-  const int s = a.s;
   double errv = 0.0, nrm2 = 0.0;
   // every lane of a SUB group has the same row and pulse, so the group is
   // active or inactive as a whole (the DPP reads stay inside it)
@@ -1046,84 +1135,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
         rs.x += dpp_d<0x140>(rs.x);
         rs.y += dpp_d<0x140>(rs.y);
       }
-      if (sub == 0) {
-        const double2 xs = (s == 0) ? a.X[idx] : a.Xs[idx];
-        const double e = a.E[row];
-        double F;
-        if (c.fmode == kFieldTable)
-          F = a.Ftab[(long long)p * c.hstride + c.step];
-        else if (c.fmode == kFieldPulse)
-          F = field_at(a.pulses[p], a.env, c.t + cC[s] * c.hc);
-        else
-          F = c.Fc;
-        double2 k;
-        if (d.gauge == 0) {  // velocity: -iE x + F (D^T x_{L-1} - D x_{L+1})
-          k.x = e * xs.y + F * (cs.x - rs.x);
-          k.y = -e * xs.x + F * (cs.y - rs.y);
-        } else {             // length: -i (E x + F (D^T x_{L-1} + D x_{L+1}))
-          const double cx = cs.x + rs.x, cy = cs.y + rs.y;
-          k.x = e * xs.y + F * cy;
-          k.y = -e * xs.x - F * cx;
-        }
-        const long long NP = a.NP;
-        a.K[s * NP + idx] = k;
-        if (c.mode != kModeRhs) {
-          const double h = c.hc;
-          const double2 x = a.X[idx];
-          if (!LAST) {
-            double vx = 1.0 * x.x, vy = 1.0 * x.y;
-            // the tableau's zero entries (22 of the 66 below the diagonal
-            // before the last row) skip their loads: a uniform branch on a
-            // constant-memory value
-            double2 kj[kStages - 1];
-#pragma unroll
-            for (int j = 0; j < kStages - 1; ++j)
-              kj[j] = (j < s && cA[s + 1][j] != 0.0) ? a.K[j * NP + idx]
-                                                     : make_double2(0.0, 0.0);
-#pragma unroll
-            for (int j = 0; j < kStages - 1; ++j) {
-              if (j < s) {
-                const double co = h * cA[s + 1][j];
-                vx += co * kj[j].x;
-                vy += co * kj[j].y;
-              }
-            }
-            const double co = h * cA[s + 1][s];
-            vx += co * k.x;
-            vy += co * k.y;
-            a.Xs[idx] = make_double2(vx, vy);
-            if (a.xs_store)
-              a.xs_store[((long long)c.step * (kStages - 1) + s) * NP + idx] =
-                  make_double2(vx, vy);
-          } else {
-            double yx = 1.0 * x.x, yy = 1.0 * x.y, ex = 0.0, ey = 0.0;
-            // k_2 .. k_5 carry zero weight in both b and b - bhat
-            double2 kj[kStages];
-#pragma unroll
-            for (int j = 0; j < kStages - 1; ++j)
-              kj[j] = (j == 0 || cB[j] != 0.0 || cDB[j] != 0.0)
-                          ? a.K[j * NP + idx]
-                          : make_double2(0.0, 0.0);
-            kj[kStages - 1] = k;
-            const double2 k1 = kj[0];
-#pragma unroll
-            for (int j = 0; j < kStages; ++j) {
-              const double cb = h * cB[j], cd = h * cDB[j];
-              yx += cb * kj[j].x;
-              yy += cb * kj[j].y;
-              ex += cd * kj[j].x;
-              ey += cd * kj[j].y;
-            }
-            a.Y[idx] = make_double2(yx, yy);
-            const double rx =
-                fabs(ex) / (c.atol + c.rtol * (fabs(x.x) + h * fabs(k1.x)));
-            const double ry =
-                fabs(ey) / (c.atol + c.rtol * (fabs(x.y) + h * fabs(k1.y)));
-            errv = rx > ry ? rx : ry;
-            nrm2 = yx * yx + yy * yy;
-          }
-        }
-      }  // sub == 0
+      if (sub == 0)
+        stage_row<LAST>(a, d, c, row, p, idx, cs, rs, errv, nrm2);
     }
   }
   if (LAST && a.red != nullptr) {
@@ -1143,6 +1156,92 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
     }
   }
 }
+// ---------------------------------------------------------------------
+// One pulse on a small system (dipoles that stay in cache): each stage in
+// one launch instead of the GEMV tile + k_stage pair.  One wave per row
+// forms both coupling sums straight from row-major copies, D_L (rows of
+// block L, row stride dr_ld) and D_L^T (rows of block L + 1, stride
+// drt_ld), lanes striding the columns (coalesced), reduces them over the
+// wave in a fixed order (DPP row sums, then the lane-16 / lane-32 swaps),
+// and lane 0 runs the stage algebra.  The stage input every wave reads is
+// not the buffer the next stage's input is written to (a.Xs_next), since
+// other waves of the launch may still be reading it.  On the last stage
+// the four rows of a workgroup leave one (error max, |y|^2) partial, as
+// k_stage's do.
+// ---------------------------------------------------------------------
+template <bool LAST>
+__global__ __launch_bounds__(256) void k_small_stage(
+    StageArgs a, SysDesc d, const double *__restrict__ Dr,
+    const long long *__restrict__ dr_off, const int *__restrict__ dr_ld,
+    const double *__restrict__ DrT, const long long *__restrict__ drt_off,
+    const int *__restrict__ drt_ld) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int row = blockIdx.x * 4 + w;
+  const Ctl &c = a.ctl[0];
+  double errv = 0.0, nrm2 = 0.0;
+  if (row < d.N && !c.done) {  // uniform over the wave
+    const int L = block_of(d, row);
+    const int i = row - d.off[L];
+    const double2 *xin = a.s == 0 ? a.X : a.Xs;
+    double2 cs = make_double2(0.0, 0.0), rs = make_double2(0.0, 0.0);
+    if (L > 0) {  // D_{L-1}^T x_{L-1}: row i of pair L - 1's transpose
+      const double *dt = DrT + drt_off[L - 1] + (long long)i * drt_ld[L - 1];
+      const double2 *xv = xin + d.off[L - 1];
+      const int n = d.off[L] - d.off[L - 1];
+      for (int j = lane; j < n; j += 64) {
+        const double v = dt[j];
+        const double2 xj = xv[j];
+        cs.x = fma(v, xj.x, cs.x);
+        cs.y = fma(v, xj.y, cs.y);
+      }
+    }
+    if (L < d.nb - 1) {  // D_L x_{L+1}: row i of pair L
+      const double *dr = Dr + dr_off[L] + (long long)i * dr_ld[L];
+      const double2 *xv = xin + d.off[L + 1];
+      const int n = d.off[L + 2] - d.off[L + 1];
+      for (int j = lane; j < n; j += 64) {
+        const double v = dr[j];
+        const double2 xj = xv[j];
+        rs.x = fma(v, xj.x, rs.x);
+        rs.y = fma(v, xj.y, rs.y);
+      }
+    }
+    cs.x = add_xor32(add_xor16(sum16(cs.x)));
+    cs.y = add_xor32(add_xor16(sum16(cs.y)));
+    rs.x = add_xor32(add_xor16(sum16(rs.x)));
+    rs.y = add_xor32(add_xor16(sum16(rs.y)));
+    if (lane == 0)
+      stage_row<LAST>(a, d, c, row, 0, (long long)row, cs, rs, errv, nrm2);
+  }
+  if (LAST && a.red != nullptr) {
+    __shared__ double se[4], sn[4];
+    if (lane == 0) {
+      se[w] = errv;
+      sn[w] = nrm2;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double em = 0.0, ns = 0.0;
+      for (int r = 0; r < 4; ++r) {
+        em = se[r] > em ? se[r] : em;
+        ns += sn[r];
+      }
+      a.red[(long long)blockIdx.x * 2 + 0] = em;
+      a.red[(long long)blockIdx.x * 2 + 1] = ns;
+    }
+  }
+}
+template __global__ void k_small_stage<false>(StageArgs, SysDesc,
+                                              const double *,
+                                              const long long *, const int *,
+                                              const double *,
+                                              const long long *, const int *);
+template __global__ void k_small_stage<true>(StageArgs, SysDesc,
+                                             const double *,
+                                             const long long *, const int *,
+                                             const double *,
+                                             const long long *, const int *);
+
 template __global__ void k_stage<1, false>(StageArgs, SysDesc);
 template __global__ void k_stage<1, true>(StageArgs, SysDesc);
 template __global__ void k_stage<kStageSub1, false>(StageArgs, SysDesc);

@@ -7,7 +7,10 @@ namespace mpci {
 
 struct StageArgs {
   const double2 *X;      // committed state (N x P)
-  double2 *Xs;           // stage input, rewritten for the next stage
+  double2 *Xs;           // stage input (stages 1..12)
+  double2 *Xs_next;      // where the next stage's input goes: Xs itself
+                         // when the coupling ran in an earlier kernel, the
+                         // other of two buffers on the fused path
   double2 *K;            // 13 stage derivatives, stride NP
   double2 *Y;            // candidate state of the attempt
   const double2 *part;   // coupling partial sums
@@ -65,6 +68,11 @@ __global__ void k_gemm_coupling(const GemmJob *jobs, const double *Dr,
                                 double *part, SysDesc d, const Ctl *ctl);
 template <int SUB, bool LAST>
 __global__ void k_stage(StageArgs a, SysDesc d);
+template <bool LAST>
+__global__ void k_small_stage(StageArgs a, SysDesc d, const double *Dr,
+                              const long long *dr_off, const int *dr_ld,
+                              const double *DrT, const long long *drt_off,
+                              const int *drt_ld);
 __global__ void k_finalize(Ctl *ctl, const double *red, int nblk, int P,
                            const double *hs);
 __global__ void k_apply(double2 *X, const double2 *Y, const Ctl *ctl,

@@ -101,6 +101,7 @@ _SIGS = [
     ("mpci_prop_gemm_config", C.c_int,
      [_vp, _ip, _ip, _ip, _dp, _ip, _ip, _ip, _ip]),
     ("mpci_prop_gemm_candidates", C.c_int, [_ip]),
+    ("mpci_prop_set_small_path", C.c_int, [_vp, C.c_int, _ip]),
     ("mpci_prop_set_gemm_candidate", C.c_int, [_vp, C.c_int]),
 ]
 

@@ -256,6 +256,15 @@ class Propagator:
         check(_lib.tdse_lib().mpci_prop_gemm_candidates(C.byref(n)))
         return n.value
 
+    def set_small_path(self, mode=-1):
+        """One pulse on a small system: fused coupling + stage launches.
+        mode -1 automatic, 0 off, 1 on where it applies.  Returns whether
+        the path is active."""
+        on = C.c_int()
+        check(_lib.tdse_lib().mpci_prop_set_small_path(self.h, int(mode),
+                                                       C.byref(on)))
+        return bool(on.value)
+
     def set_gemm_candidate(self, index):
         """Pin candidate tile `index` in place of the autotuned one."""
         check(_lib.tdse_lib().mpci_prop_set_gemm_candidate(self.h, int(index)))

@@ -148,6 +148,7 @@ def test_persistent_gemv_grid(mpci, oracle, grid):
     g, o = build(mpci, oracle, [300, 420, 270, 150], "v", seed=12)
     g.set_gemv_grid(grid)
     prop = mpci.Propagator(g)
+    assert not prop.set_small_path(0)  # the tile kernel, not the fused path
     rng = np.random.default_rng(grid)
     x = rng.standard_normal(2 * g.N)
     got = prop.rhs(0.37, x)
@@ -229,3 +230,63 @@ def test_reference_he_test_shape_two_steps(mpci, oracle, gauge):
     batch.prop_fixed(5.0, 0.1, np.vstack([F, F]))
     for p in range(2):
         assert np.abs(batch.get_state(p) - xr).max() < 1e-12, p
+
+
+@pytest.mark.parametrize("gauge", ["v", "l"])
+@pytest.mark.parametrize("s", [[3, 2, 4], [1, 1], [300, 257, 513],
+                               [198, 198, 198, 198], [1, 700, 2]])
+def test_small_path_against_tile_path(mpci, oracle, gauge, s):
+    """One pulse on a small system: the fused small-system stage kernel and
+    the GEMV tile + stage kernels give the same RHS (to 1e-13 of max|k|,
+    the coupling sums run in another order) and the same fixed-step
+    propagation, both against the oracle; the automatic choice takes the
+    fused path for these shapes.  The propagation bar is 4e-12 after 25
+    steps: rounding of ~5e-14 per step (13 stages of sums over up to 513
+    terms) accumulates, against the 1e-12 of the two-step tests."""
+    from mpci import synth
+    g, o = build(mpci, oracle, s, gauge, seed=sum(s))
+    rng = np.random.default_rng(len(s))
+    x = rng.standard_normal(2 * g.N)
+    ref = o.rhs(-0.61, x)
+    bar = 1e-13 * max(1.0, np.abs(ref).max())
+    small = mpci.Propagator(g)
+    tile = mpci.Propagator(g)
+    assert small.set_small_path(-1) == (len(s) > 1)
+    assert not tile.set_small_path(0)
+    assert np.abs(small.rhs(-0.61, x) - ref).max() <= bar
+    assert np.abs(tile.rhs(-0.61, x) - ref).max() <= bar
+    pulse = oracle.Pulse.sine(0.05 ** 2, 0.5, 3, gauge)
+    F = fixed_fields(pulse, 0.0, 0.1, 25)
+    x0 = synth.ground_state(g.N)
+    xr, _, _, _ = o.prop_fixed(pulse, 0.0, 0.1, 25, x0)
+    got = []
+    for prop in (small, tile):
+        prop.set_state(x0)
+        prop.prop_fixed(0.0, 0.1, F)
+        got.append(prop.get_state())
+        assert np.abs(got[-1] - xr).max() < 4e-12
+    assert np.abs(got[0] - got[1]).max() < 4e-12
+
+
+def test_small_path_adaptive_and_graph(mpci, oracle):
+    """The fused path under adaptive control, captured and eager: the
+    oracle's accepted/rejected counts and its state within 1e-10, and the
+    captured and eager runs bitwise equal."""
+    from mpci import synth
+    g, o = build(mpci, oracle, [198] * 4, "v", seed=41)
+    x0 = synth.ground_state(g.N)
+    T = 2 * 2 * math.pi / 0.5
+    po = oracle.Pulse.sine(0.05 ** 2, 0.5, 2)
+    xr, sr = o.prop_adaptive(po, 0.0, T, 0.05, 1e-10, 1e-10, x0)[:2]
+    out = []
+    for graph in (True, False):
+        prop = mpci.Propagator(g)
+        assert prop.set_small_path(1)
+        prop.set_graph(graph)
+        prop.set_pulse(mpci.Pulse.sine(0.05 ** 2, 0.5, 2))
+        prop.set_state(x0)
+        st, _ = prop.prop_adaptive(0.0, T, 0.05, 1e-10, 1e-10)
+        out.append(prop.get_state())
+        assert st[0].accepted == sr.accepted and st[0].rejected == sr.rejected
+        assert np.abs(out[-1] - xr).max() < 1e-10
+    assert np.array_equal(out[0], out[1])
