@@ -191,7 +191,7 @@ int mpci_prop_set_graph(mpci_prop *prop, int enable);
  * from row-major copies of D_L and D_L^T, one wave per row, then the stage
  * algebra) instead of the GEMV tile kernel and the stage kernel.  mode -1
  * (default): on when P = 1, the system has more than one block and its
- * dipoles take at most 8 MiB; 0: off; 1: on whenever P = 1 and there is
+ * dipoles take at most 12 MiB; 0: off; 1: on whenever P = 1 and there is
  * more than one block.  active (may be NULL) reports the outcome.  Both
  * paths give the same result to rounding (the coupling sums in another
  * order). */

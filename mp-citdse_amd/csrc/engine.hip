@@ -746,8 +746,10 @@ static int autotune_gemm(mpci_prop *pr) {
 
 // Dipole bytes (8 S) up to which one pulse takes the small-system path by
 // default: there every stage is a handful of microseconds of launch and
-// latency, and the fused kernel halves the launches per attempt.
-constexpr long long kSmallPathBytes = 8LL << 20;
+// latency, and the fused kernel halves the launches per attempt.  Measured
+// on four equal blocks: 1.95x the tile path at 0.9 MB of dipoles, 1.30x at
+// 8.6 MB, even at 15 MB, 0.85x at 24 MB.
+constexpr long long kSmallPathBytes = 12LL << 20;
 
 // mode -1: automatic (one pulse, more than one block, 8 S within
 // kSmallPathBytes); 0: off; 1: on wherever it applies (one pulse, more
