@@ -224,3 +224,42 @@ def test_pinned_candidate_in_captured_steps(mpci, oracle):
     for p in range(P):
         xr, _, _, _ = o.prop_fixed(pulses[p], 0.0, dt, steps, x0, track=0)
         assert np.abs(prop.get_state(p) - xr).max() < 1e-12, p
+
+
+def test_full_he_shape_batch64(mpci, oracle):
+    """The batch-64 workload at its real shape (4 blocks of 4000 states,
+    velocity gauge, P = 64): the autotuned MFMA coupling against the
+    oracle's RHS on a sample of pulses, then three adaptive attempts of the
+    batch against single-pulse runs of the same pulses (same accepted
+    counts, states within 1e-11)."""
+    from mpci import synth
+    g, o = build(mpci, oracle, [4000] * 4, "v", seed=20261202)
+    P = 64
+    prop = mpci.Propagator(g, P=P)
+    rng = np.random.default_rng(64)
+    x = rng.standard_normal((P, 2 * g.N))
+    F = rng.uniform(-0.05, 0.05, P)
+    got = prop.rhs(F, x.reshape(-1)).reshape(P, -1)
+    for p in (0, 21, 42, 63):
+        ref = o.rhs(F[p], x[p])
+        assert np.abs(got[p] - ref).max() <= 1e-13 * max(1, np.abs(ref).max()), p
+    w, E0, cyc = synth.pulse_batch(P, seed=3)
+    x0 = synth.ground_state(g.N)
+    t1 = np.array([mpci.period(w[p], int(cyc[p])) for p in range(P)])
+    for p in range(P):
+        prop.set_state(x0, p)
+        prop.set_pulse(mpci.Pulse.sine(E0[p] ** 2, w[p], int(cyc[p])), p)
+    st, _ = prop.prop_adaptive(0.0, t1, 0.05, 1e-10, 1e-10, max_attempts=3,
+                               allow_limit=True)
+    for p in (0, 21, 42, 63):
+        one = mpci.Propagator(g)
+        one.set_state(x0)
+        one.set_pulse(mpci.Pulse.sine(E0[p] ** 2, w[p], int(cyc[p])))
+        s1, _ = one.prop_adaptive(0.0, t1[p], 0.05, 1e-10, 1e-10,
+                                  max_attempts=3, allow_limit=True)
+        assert s1[0].attempts == st[p].attempts == 3
+        assert s1[0].accepted == st[p].accepted
+        assert abs(s1[0].t - st[p].t) < 1e-12
+        assert np.abs(one.get_state() - prop.get_state(p)).max() < 1e-11, p
+        one.close()
+    prop.close()

@@ -91,3 +91,29 @@ def test_multi_attempt_limit(mpci, members):
         mpci.multi_adaptive([fresh(mpci, members[0])], [0.0],
                             [members[0]["t1"]], [0.05], TOL, TOL,
                             max_attempts=7)
+
+
+def test_multi_full_shapes(mpci):
+    """The uneven multi-instance workload at its real shapes: 32 pulses on
+    an H-shaped system of 21 blocks of 998 states and 32 on the He shape
+    (4 blocks of 4000), three attempts each, run concurrently; every member
+    equals its own separate run bit for bit."""
+    ms = [member(mpci, [998] * 21, "v", 51, 32, 61, 20),
+          member(mpci, [4000] * 4, "v", 20261202, 32, 62, 20)]
+    props = [fresh(mpci, m) for m in ms]
+    st = mpci.multi_adaptive(props, [0.0] * 2, [m["t1"] for m in ms],
+                             [0.05] * 2, TOL, TOL, max_attempts=3,
+                             allow_limit=True)
+    for i, m in enumerate(ms):
+        one = fresh(mpci, m)
+        s1, _ = one.prop_adaptive(0.0, m["t1"], 0.05, TOL, TOL,
+                                  max_attempts=3, allow_limit=True)
+        for p in range(m["P"]):
+            assert st[i][p].attempts == s1[p].attempts == 3, (i, p)
+            assert st[i][p].accepted == s1[p].accepted, (i, p)
+            assert st[i][p].t == s1[p].t, (i, p)
+            assert np.array_equal(props[i].get_state(p), one.get_state(p)), \
+                (i, p)
+        one.close()
+    for pr in props:
+        pr.close()
