@@ -491,6 +491,19 @@ def copy_calibration(n_bytes=1 << 30, reps=3):
             "kernel": "torch elementwise mul (copy)"}
 
 
+def measured_ceilings(mpci, stream_bytes):
+    """The ceilings this device reaches in practice, beside the published
+    peaks: a non-temporal streaming read over a buffer the size of the
+    dipole store, and back-to-back FP64 MFMA with no memory traffic."""
+    return {"hbm_read_gbs": mpci.probe_hbm_read(stream_bytes, 20),
+            "hbm_probe": f"non-temporal 16-byte read of {stream_bytes} bytes "
+                         "(the dipole store's size), best of 2/4/8 "
+                         "workgroups per CU",
+            "mfma_f64_tflops": mpci.probe_mfma_f64(16384),
+            "mfma_probe": "v_mfma_f64_16x16x4f64, 4 independent chains per "
+                          "wave, 2 waves per SIMD, no memory traffic"}
+
+
 def traffic_from_profiles():
     p = os.path.join(ROOT, "profiles", "k_gemv_tile_he4x4000_traffic.json")
     try:
@@ -542,6 +555,8 @@ def main():
         f"{r['wall']:.3f} s")
     avg_us = r["kernel_ms"] / max(r["kernel_launches"], 1) * 1e3
     achieved = r["kernel_bytes"] / (avg_us * 1e-6) / 1e9
+    log("bench: measured ceilings (HBM read, FP64 MFMA) ...")
+    ceil = measured_ceilings(mpci, r["kernel_bytes"])
     ref = None
     if not args.no_reference_mode:
         log("bench: reference mode ...")
@@ -570,6 +585,10 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(),
+                     "measured_peak": round(ceil["hbm_read_gbs"], 1),
+                     "frac_of_measured": round(
+                         achieved / ceil["hbm_read_gbs"], 4),
+                     "measured_peak_probe": ceil["hbm_probe"],
                      "avg_launch_us": round(avg_us, 3),
                      "bytes_per_launch": r["kernel_bytes"],
                      "launches": r["kernel_launches"]},
@@ -588,6 +607,10 @@ def main():
         b["per_pulse_speedup_vs_single"] = b["pulse_steps_per_s"] / value
         if b["mfma_tflops"] is not None:
             b["mfma_frac"] = b["mfma_tflops"] / 78.6
+            b["mfma_measured_peak_tflops"] = ceil["mfma_f64_tflops"]
+            b["mfma_frac_of_measured"] = (b["mfma_tflops"] /
+                                          ceil["mfma_f64_tflops"])
+            b["mfma_probe"] = ceil["mfma_probe"]
         out["batch64"] = {k: (round(v, 4) if isinstance(v, float) else v)
                           for k, v in b.items()}
     if not args.no_multi:

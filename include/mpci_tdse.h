@@ -222,6 +222,19 @@ int mpci_prop_gemm_config(mpci_prop *prop, int *wm, int *wn, int *ks,
 int mpci_prop_gemm_candidates(int *count);
 int mpci_prop_set_gemm_candidate(mpci_prop *prop, int index);
 
+/* ---- measurement probes --------------------------------------------- */
+/* No counterpart in the reference: the ceilings the current device reaches
+ * in practice, reported beside the published peaks by the benchmark.
+ * mpci_probe_hbm_read: a buffer of `bytes` (>= 1 MiB) streamed once per
+ * launch with 16-byte non-temporal loads, `reps` timed launches at 2, 4 and
+ * 8 workgroups per CU; the best rate in GB/s (1e9 bytes/s).
+ * mpci_probe_mfma_f64: v_mfma_f64_16x16x4f64 back to back (four
+ * independent chains per wave, two waves per SIMD, `iters` >= 16 steps per
+ * chain, no memory traffic); the best of three launches in TFLOP/s.
+ * Both allocate their own device memory and use the null stream. */
+int mpci_probe_hbm_read(long long bytes, int reps, double *gbps);
+int mpci_probe_mfma_f64(int iters, double *tflops);
+
 #ifdef __cplusplus
 }
 #endif

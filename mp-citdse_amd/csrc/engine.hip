@@ -1773,3 +1773,103 @@ extern "C" int mpci_prop_set_small_path(mpci_prop *pr, int mode,
   if (active) *active = pr->small ? 1 : 0;
   return MPCI_OK;
 }
+
+// ---------------------------------------------------------------------
+// measurement probes (no counterpart in the reference): the ceilings this
+// device reaches, beside the published peaks
+// ---------------------------------------------------------------------
+namespace {
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  int create() {
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    return MPCI_OK;
+  }
+  // milliseconds between the two events, after both have completed
+  int elapsed(float *ms) {
+    HIPCHK(hipEventSynchronize(b));
+    HIPCHK(hipEventElapsedTime(ms, a, b));
+    return MPCI_OK;
+  }
+};
+
+int device_cus() {
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                            dev) != hipSuccess || ncu <= 0)
+    return 256;
+  return ncu;
+}
+}  // namespace
+
+extern "C" int mpci_probe_hbm_read(long long bytes, int reps, double *gbps) {
+  if (!gbps || bytes < (1LL << 20) || reps < 1)
+    return fail(MPCI_ERR_ARG, "mpci_probe_hbm_read: need gbps, bytes >= "
+                              "1 MiB and reps >= 1");
+  *gbps = 0.0;
+  const long long n = bytes / 16 / 1024 * 1024;  // double2, whole grid steps
+  DevBuf<double2> buf;
+  DevBuf<double> out;
+  TRY(buf.alloc((size_t)n));
+  TRY(out.alloc(1));
+  HIPCHK(hipMemset(buf.p, 0, (size_t)n * 16));
+  EventPair ev;
+  TRY(ev.create());
+  const int ncu = device_cus();
+  double best = 0.0;
+  for (int per_cu : {2, 4, 8}) {
+    const dim3 grid((unsigned)(per_cu * ncu));
+    for (int r = 0; r < 2; ++r)
+      hipLaunchKernelGGL(k_probe_read, grid, dim3(256), 0, 0, buf.p, n,
+                         out.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev.a, 0));
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_probe_read, grid, dim3(256), 0, 0, buf.p, n,
+                         out.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev.b, 0));
+    float ms = 0.f;
+    TRY(ev.elapsed(&ms));
+    if (ms > 0.f)
+      best = std::max(best, (double)n * 16 * reps / (ms * 1e-3) / 1e9);
+  }
+  *gbps = best;
+  return MPCI_OK;
+}
+
+extern "C" int mpci_probe_mfma_f64(int iters, double *tflops) {
+  if (!tflops || iters < 16)
+    return fail(MPCI_ERR_ARG, "mpci_probe_mfma_f64: need tflops and "
+                              "iters >= 16");
+  *tflops = 0.0;
+  DevBuf<double> out;
+  TRY(out.alloc(1));
+  EventPair ev;
+  TRY(ev.create());
+  // two workgroups of four waves per CU: two waves per SIMD, four
+  // independent accumulator chains each
+  const int ncu = device_cus();
+  const dim3 grid((unsigned)(2 * ncu));
+  const double flops = (double)grid.x * 4 * 4 * iters * (16.0 * 16 * 4 * 2);
+  hipLaunchKernelGGL(k_probe_mfma, grid, dim3(256), 0, 0, out.p, iters);
+  HIPCHK(hipGetLastError());
+  double best = 0.0;
+  for (int r = 0; r < 3; ++r) {
+    HIPCHK(hipEventRecord(ev.a, 0));
+    hipLaunchKernelGGL(k_probe_mfma, grid, dim3(256), 0, 0, out.p, iters);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev.b, 0));
+    float ms = 0.f;
+    TRY(ev.elapsed(&ms));
+    if (ms > 0.f) best = std::max(best, flops / (ms * 1e-3) / 1e12);
+  }
+  *tflops = best;
+  return MPCI_OK;
+}

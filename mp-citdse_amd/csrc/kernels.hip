@@ -1531,4 +1531,50 @@ __global__ __launch_bounds__(256) void k_weighted_pop(
   if (t == 0) out[o * P + p] = s[0];
 }
 
+// ---------------------------------------------------------------------
+// Measurement probes: the ceilings this device reaches in practice, which
+// the roofline fractions are also quoted against.  k_probe_read streams a
+// buffer once with 16-byte non-temporal loads, four in flight per lane (n
+// a multiple of 1024 double2).  k_probe_mfma keeps every FP64 matrix pipe
+// busy with four independent v_mfma_f64_16x16x4f64 chains per wave and no
+// memory traffic.  Neither result is used; the final comparison only keeps
+// the work from being removed.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_probe_read(
+    const double2 *__restrict__ a, long long n, double *out) {
+  typedef double dv2 __attribute__((ext_vector_type(2)));
+  const dv2 *p = reinterpret_cast<const dv2 *>(a);
+  const long long stride = (long long)gridDim.x * 1024;
+  double s0 = 0.0, s1 = 0.0;
+  for (long long i = (long long)blockIdx.x * 1024 + threadIdx.x; i + 768 < n;
+       i += stride) {
+    dv2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      v[u] = __builtin_nontemporal_load(p + i + 256 * u);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      s0 += v[u].x;
+      s1 += v[u].y;
+    }
+  }
+  if (s0 + s1 == 1.2345678e300) out[0] = s0;
+}
+
+// (the occupancy hint keeps the accumulators in the VGPR form of the
+// instruction; without it they live in AGPRs and are copied every trip)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_probe_mfma(double *out, int iters) {
+  const int lane = threadIdx.x & 63;
+  const double a = 1.0 + 1e-3 * lane, b = 1.0 - 1e-3 * lane;
+  dmf4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = c0, c2 = c0, c3 = c0;
+  for (int k = 0; k < iters; ++k) {
+    c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c0, 0, 0, 0);
+    c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c1, 0, 0, 0);
+    c2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c2, 0, 0, 0);
+    c3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c3, 0, 0, 0);
+  }
+  if (c0[0] + c1[1] + c2[2] + c3[3] == 1.2345678e300) out[0] = c0[0];
+}
+
 }  // namespace mpci

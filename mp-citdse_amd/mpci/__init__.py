@@ -6,4 +6,5 @@ that interface.
 """
 from ._lib import MpciError, EXPORTED  # noqa: F401
 from .engine import (Propagator, Pulse, System, device_count,  # noqa: F401
-                     multi_adaptive, period, to_au)
+                     multi_adaptive, period, probe_hbm_read, probe_mfma_f64,
+                     to_au)

@@ -103,6 +103,8 @@ _SIGS = [
     ("mpci_prop_gemm_candidates", C.c_int, [_ip]),
     ("mpci_prop_set_small_path", C.c_int, [_vp, C.c_int, _ip]),
     ("mpci_prop_set_gemm_candidate", C.c_int, [_vp, C.c_int]),
+    ("mpci_probe_hbm_read", C.c_int, [C.c_longlong, C.c_int, _dp]),
+    ("mpci_probe_mfma_f64", C.c_int, [C.c_int, _dp]),
 ]
 
 # Every symbol the header declares; tests check the library exports them.

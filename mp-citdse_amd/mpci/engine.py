@@ -304,3 +304,20 @@ def device_count() -> int:
     n = C.c_int()
     check(_lib.tdse_lib().mpci_device_count(C.byref(n)))
     return n.value
+
+
+def probe_hbm_read(nbytes: int = 384 << 20, reps: int = 20) -> float:
+    """Best HBM read rate in GB/s of a non-temporal streaming kernel over
+    nbytes (mpci_probe_hbm_read): the ceiling reached in practice."""
+    v = C.c_double()
+    check(_lib.tdse_lib().mpci_probe_hbm_read(int(nbytes), int(reps),
+                                              C.byref(v)))
+    return v.value
+
+
+def probe_mfma_f64(iters: int = 4096) -> float:
+    """Best FP64 MFMA rate in TFLOP/s of back-to-back v_mfma_f64_16x16x4f64
+    chains with no memory traffic (mpci_probe_mfma_f64)."""
+    v = C.c_double()
+    check(_lib.tdse_lib().mpci_probe_mfma_f64(int(iters), C.byref(v)))
+    return v.value

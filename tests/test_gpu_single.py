@@ -290,3 +290,20 @@ def test_small_path_adaptive_and_graph(mpci, oracle):
         assert st[0].accepted == sr.accepted and st[0].rejected == sr.rejected
         assert np.abs(out[-1] - xr).max() < 1e-10
     assert np.array_equal(out[0], out[1])
+
+
+def test_measurement_probes(mpci):
+    """The ceiling probes: a 1 GiB streaming read lands between a sanity
+    floor and the 8 TB/s published HBM peak (with room for cache effects);
+    back-to-back FP64 MFMA stays at or below the 78.6 TFLOP/s published
+    peak; arguments outside the documented ranges are refused."""
+    bw = mpci.probe_hbm_read(1 << 30, 3)
+    assert 500.0 < bw < 8000.0 * 1.25, bw
+    tf = mpci.probe_mfma_f64(1024)
+    assert 5.0 < tf <= 78.6 * 1.01, tf
+    for bad in ((100, 1), (1 << 20, 0)):
+        with pytest.raises(mpci.MpciError) as e:
+            mpci.probe_hbm_read(*bad)
+        assert e.value.code == -1
+    with pytest.raises(mpci.MpciError):
+        mpci.probe_mfma_f64(8)
