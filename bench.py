@@ -538,7 +538,7 @@ def main():
                          "the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build propagates on a "
+        print("bench.py measures one GPU only: this implementation runs on a "
               "single MI355X (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 

@@ -2,8 +2,8 @@
 its d1e stage (one-electron dipole blocks).
 
 TEST INFRASTRUCTURE ONLY: only tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg may use it, as the checker of the GPU path
-(lib/libmpci_ci.so), never as the product.
+bench.py's cpu_baseline leg may use it, as the oracle the GPU path
+(lib/libmpci_ci.so) is compared with, never as the product.
 
 Follows src/w2e/w2e.cpp:17-125:
   * formCIh0 (w2e.cpp:17-82): H = diag(e_2e) + V_12, V_12 read as the packed

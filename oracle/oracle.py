@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker.  The product path
+cpu_baseline leg of bench.py, always as the oracle.  The product path
 (mp-citdse_amd/) never imports this module.
 
 The oracle restates MP-CITDSE's `tdse` propagation path

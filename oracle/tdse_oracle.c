@@ -1,7 +1,7 @@
 /*
  * tdse_oracle.c -- CPU restatement of MP-CITDSE's `tdse` hot path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker, never the product): loaded by
+ * TEST INFRASTRUCTURE ONLY (the oracle, never the product): loaded by
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
  * See tdse_oracle.h for the reference file:line each function follows.
  */

@@ -1,7 +1,7 @@
 /*
  * tdse_oracle.h -- CPU restatement of MP-CITDSE's `tdse` propagation path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle for the HIP
  * implementation: only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it.  The product never links or calls it.
  *

@@ -1,8 +1,8 @@
 """CPU check of the gfx950 assembly of every kernel source: device code
 writes memory through vector instructions only (no scalar-unit stores,
-atomics or scalar-cache write-backs).  This file names those instructions
-to look for them, so it is kept off GPU runs (.gpurunignore); it compiles
-the sources on the CPU and runs nothing on a device."""
+atomics or scalar-cache write-backs).  It compiles the sources on the CPU
+and runs nothing on a device, so it never needs to travel with the GPU
+tests."""
 import os
 import re
 import shutil
