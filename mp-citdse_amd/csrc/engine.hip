@@ -272,12 +272,16 @@ static int build_system(mpci_system *sys, const double *const *E,
     sys->ncb[L] = (sys->s[L + 1] + kGvTC - 1) / kGvTC;
     sys->S += (long long)sys->s[L] * sys->s[L + 1];
     job0[L] = (int)sys->jobs.size();
+    // Column blocks in an order rotated by the row block: workgroup w
+    // streams units w, w + G, ..., so without the rotation the narrow last
+    // column block of every row block would fall to the workgroups with
+    // w = ncb - 1 (mod ncb), all on one XCD when ncb is a multiple of 8.
     for (int rb = 0; rb < sys->nrb[L]; ++rb)
-      for (int cb = 0; cb < sys->ncb[L]; ++cb) {
+      for (int k = 0; k < sys->ncb[L]; ++k) {
         TileJob j;
         j.L = L;
         j.rb = rb;
-        j.cb = cb;
+        j.cb = (k + rb) % sys->ncb[L];
         j.pad = 0;
         j.doff = (long long)sys->jobs.size();
         sys->jobs.push_back(j);

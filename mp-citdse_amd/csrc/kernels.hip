@@ -96,7 +96,9 @@ __global__ void k_relayout_gemv(const double *__restrict__ D, int sL, int sL1,
   if (e >= nelem) return;
   long long tile = e / kGvTileElems;
   int within = (int)(e - tile * kGvTileElems);
-  int rb = (int)(tile / ncb), cb = (int)(tile % ncb);
+  // unit order within a pair: row blocks in turn, column blocks rotated by
+  // the row block (the same order as the job list)
+  int rb = (int)(tile / ncb), cb = (int)((tile % ncb + rb) % ncb);
   int el = within & 1;
   int q = within >> 1;
   int t = q % kGvThreads;
