@@ -201,3 +201,56 @@ def test_cli_1e_gaussian_length(tmp_path, oracle):
     assert np.abs(x - xr).max() < 1e-12
     pop = np.loadtxt(tmp_path / "out" / "h_pop21.dat", comments="#")
     assert np.abs(pop[1:, 1] - popr).max() < 1e-12
+
+
+def test_cli_golden_he_input(tmp_path, oracle):
+    """The reference's own inp/He_test.yaml (tests/golden), unchanged,
+    through the drop-in on tdse_gen inputs of its block sizes (3030, 2799,
+    2808, 2808; velocity gauge; w = 15 eV, 1e14 W/cm^2, 12 cycles: 1367
+    reference-mode steps).  Checked: the coefficient-file cadence; the
+    state after the first step against the oracle to the files' six
+    significant digits; the 16-digit population file against the library
+    path driven with the same frozen fields; the final norm."""
+    import shutil
+
+    import mpci
+    states = [3030, 2799, 2808, 2808]
+    shutil.copy(os.path.join(ROOT, "tests", "golden", "He_test.yaml"),
+                tmp_path / "He_test.yaml")
+    subprocess.run([os.path.join(BIN, "tdse_gen"), "-e", "2", "-p", "he",
+                    "-s", ",".join(map(str, states)), "-g", "v"],
+                   cwd=tmp_path, check=True)
+    run(["-e", "2", "-f", "He_test.yaml", "-o", "out"], tmp_path)
+    out = tmp_path / "out"
+    w = 15 * EV
+    steps = int(12 * 2.0 * math.pi / w / 0.1)
+    assert steps == 1367
+    pr, t, want = steps // 10, 0.0, {"%f" % 0.0}
+    F = np.empty(steps)
+    pulse = oracle.Pulse.sine(1.0e14 * IO, w, 12, gauge="v")
+    for st in range(steps):
+        F[st] = pulse(t + 0.1)
+        t += 0.1
+        if st % pr == 0:
+            want.add("%f" % t)
+    want.add("%f" % t)
+    got = {f[len("he_ct_"):-4] for f in os.listdir(out)
+           if f.startswith("he_ct_")}
+    assert got == want
+    E, D = leading(states, states)
+    o = oracle.System(E, D, gauge="v")
+    x0 = np.zeros(2 * o.N)
+    x0[0] = 1.0
+    x1 = o.step(F[0], x0, 0.1)
+    x1 /= math.sqrt(np.dot(x1, x1))
+    x = read_state(out / ("he_ct_%f.dat" % 0.1), o.N)
+    assert np.all(np.abs(x - x1) <= 1e-5 * np.abs(x1) + 1e-14)
+    prop = mpci.Propagator(mpci.System(E, D, gauge="v"))
+    prop.set_state(x0)
+    popl = prop.prop_fixed(0.0, 0.1, F[None, :], track=0)[0]
+    pop = np.loadtxt(out / "he_pop10.dat", comments="#")
+    assert pop.shape == (steps + 1, 2)
+    assert np.abs(pop[1:, 1] - popl).max() < 1e-15
+    xf = read_state(out / ("he_ct_%f.dat" % t), o.N)
+    assert abs(np.dot(xf, xf) - 1.0) < 1e-4
+    prop.close()
