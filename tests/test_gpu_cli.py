@@ -254,3 +254,45 @@ def test_cli_golden_he_input(tmp_path, oracle):
     xf = read_state(out / ("he_ct_%f.dat" % t), o.N)
     assert abs(np.dot(xf, xf) - 1.0) < 1e-4
     prop.close()
+
+
+def test_cli_golden_h_input(tmp_path, oracle):
+    """The reference's own inp/H_test.yaml (tests/golden), unchanged,
+    through the 1e drop-in (`tdse -e 1`) on tdse_gen inputs of its block
+    sizes (l <= 5, 300 states each; velocity gauge; w = 8.5 eV,
+    1e13 W/cm^2, 12 cycles: 2413 reference-mode steps).  Checked: the
+    coefficient-file cadence, the 16-digit ground-population file against
+    the oracle over the first 250 steps (within 1e-12), and the final
+    norm."""
+    import shutil
+    states = [300] * 6
+    shutil.copy(os.path.join(ROOT, "tests", "golden", "H_test.yaml"),
+                tmp_path / "H_test.yaml")
+    subprocess.run([os.path.join(BIN, "tdse_gen"), "-e", "1", "-p", "h",
+                    "-s", ",".join(map(str, states)), "-g", "v"],
+                   cwd=tmp_path, check=True)
+    run(["-e", "1", "-f", "H_test.yaml", "-o", "out"], tmp_path)
+    out = tmp_path / "out"
+    w = 8.5 * EV
+    steps = int(12 * 2.0 * math.pi / w / 0.1)
+    assert steps == 2413
+    pr, t, want = steps // 10, 0.0, {"%f" % 0.0}
+    for st in range(steps):
+        t += 0.1
+        if st % pr == 0:
+            want.add("%f" % t)
+    want.add("%f" % t)
+    got = {f[len("h_ct_"):-4] for f in os.listdir(out)
+           if f.startswith("h_ct_")}
+    assert got == want
+    E, D = leading(states, states)
+    o = oracle.System(E, D, gauge="v")
+    x0 = np.zeros(2 * o.N)
+    x0[0] = 1.0
+    pulse = oracle.Pulse.sine(1.0e13 * IO, w, 12, gauge="v")
+    _, _, popr, _ = o.prop_fixed(pulse, 0.0, 0.1, 250, x0, track=0)
+    pop = np.loadtxt(out / "h_pop10.dat", comments="#")
+    assert pop.shape == (steps + 1, 2)
+    assert np.abs(pop[1:251, 1] - popr).max() < 1e-12
+    xf = read_state(out / ("h_ct_%f.dat" % t), o.N)
+    assert abs(np.dot(xf, xf) - 1.0) < 1e-4
