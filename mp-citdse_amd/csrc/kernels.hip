@@ -86,7 +86,7 @@ __device__ __forceinline__ double field_at(const PulseDev &p,
 // tile is stored next to row group i of all other tiles (the workgroups
 // that stream them run concurrently, so HBM sees one contiguous front):
 // element (i, job, j, t, e) sits at (((i * njobs + job) * 4 + j) * 512 + t)
-// * 2 + e and holds D[rb*256 + (t&15) + 16 i][cb*256 + 2((t>>4) + 32 j) + e]
+// * 2 + e and holds D[rb*128 + (t&15) + 16 i][cb*256 + 2((t>>4) + 32 j) + e]
 // of the job's tile (rb, cb), or 0 outside the block.
 // ---------------------------------------------------------------------
 __global__ void k_relayout_gemv(const double *__restrict__ D, int sL, int sL1,
