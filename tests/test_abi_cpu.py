@@ -66,3 +66,41 @@ def test_synthetic_generator_is_seeded():
         assert np.all(np.diff(e) >= 0) and e.min() >= -2.9 and e.max() < 5.0
     d = synth.system_arrays([400, 400], seed=1)[1][0]
     assert abs(d.std() * np.sqrt(400) - 1.0) < 0.01 and abs(d.mean()) < 2e-3
+
+
+def test_system_create_refuses_bad_arguments():
+    """mpci_system_create checks its arguments before any device call: a
+    block count outside 1..32, an unknown gauge, an empty block, a NULL
+    energy or dipole block, and a state beyond 2^30 coefficients are each
+    refused with MPCI_ERR_ARG and a message, and no system is returned.
+    The pointers are never read on these paths."""
+    from mpci import _lib
+    lib = ctypes.CDLL(_lib.TDSE_LIB)
+    f = lib.mpci_system_create
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_char,
+                  ctypes.POINTER(ctypes.c_void_p),
+                  ctypes.POINTER(ctypes.c_void_p),
+                  ctypes.POINTER(ctypes.c_void_p)]
+    lib.mpci_last_error.restype = ctypes.c_char_p
+    buf = (ctypes.c_double * 4)()
+    ok = ctypes.addressof(buf)
+
+    def call(s, gauge=b"v", e_null=-1, d_null=-1, nb=None):
+        n = len(s) if nb is None else nb
+        m = max(len(s), 1)
+        S = (ctypes.c_int * m)(*(s or [1]))
+        E = (ctypes.c_void_p * m)(*[None if i == e_null else ok
+                                    for i in range(m)])
+        D = (ctypes.c_void_p * m)(*[None if i == d_null else ok
+                                    for i in range(m)])
+        out = ctypes.c_void_p()
+        rc = f(n, S, gauge, E, D, ctypes.byref(out))
+        return rc, lib.mpci_last_error().decode(), out.value
+
+    cases = [call([], nb=0), call([4] * 33), call([4, 4], gauge=b"x"),
+             call([4, 0, 4]), call([4, 4], e_null=1), call([4, 4], d_null=0),
+             call([(1 << 29) + 1, (1 << 29) + 1])]
+    for rc, msg, out in cases:
+        assert rc == -1 and msg and out is None, (rc, msg)
+    assert "too large" in cases[-1][1]
