@@ -64,7 +64,16 @@ typedef struct mpci_adjoint_stats {
   int segments;    /* checkpoint segments swept backward */
   double fwd_ms;   /* forward pass (adaptive, with checkpoints) */
   double bwd_ms;   /* segment recomputation + backward sweep */
+  long long peak_bytes;  /* most device memory the call's own buffers held */
+  int ckpt_slots;  /* checkpoint slots allocated (grown with the run) */
 } mpci_adjoint_stats;
+
+/* Rows of the observable weights (mpci_obs_weights). */
+#define MPCI_OBS_GROUND 0    /* ground-state population |c_0|^2 */
+#define MPCI_OBS_YIELD 1     /* ionisation yield */
+#define MPCI_OBS_EXCITED 2   /* excited population (two electrons) */
+#define MPCI_OBS_NORM 3      /* sum_k |c_k|^2 */
+#define MPCI_OBS_ROWS 4
 
 typedef struct mpci_system mpci_system;
 typedef struct mpci_prop mpci_prop;
@@ -182,6 +191,33 @@ int mpci_prop_adjoint(mpci_prop *prop, const double *t0, const double *t1,
  * |c_{k,p}|^2 (ground population, yield and excitation masks of
  * pes::genPES2e, src/pes/pes.cpp:199-213; all-ones for the norm). */
 int mpci_prop_populations(mpci_prop *prop, int nobs, const double *W,
+                          double *out);
+
+/* ---- observables: src/pes/pes.cpp ------------------------------------ */
+/* The scalars pes prints, as rows of per-state weights: observable
+ * o = sum_k W[o * N + k] |c_k|^2, N = sum of s, rows MPCI_OBS_*.
+ *   GROUND   |c_0|^2 (pes.cpp:145, 211-213).
+ *   YIELD    electrons 2 (genPES2e, pes.cpp:199-205): the states i in
+ *            [1, s_L - 2] of every block L with E_L[i] - threshold > 0,
+ *            threshold = the lowest one-electron l = 0 energy, which pes
+ *            reads from dat/<pot>0.h5:/En[0] (pes.cpp:168-176).
+ *            electrons 1 (genPES1e, pes.cpp:90-110): the states i >= 1 of
+ *            every block whose energy falls in a bin k in [1, s_0 - 2] of
+ *            the l = 1 energies (bin k spans (E_1[k] - |E_1[k]-E_1[k-1]|/2,
+ *            E_1[k] + |E_1[k+1]-E_1[k]|/2), the first such bin counts) with
+ *            E_1[k] > 0; threshold is unused; needs nb >= 2, s_0 <= s_1.
+ *   EXCITED  electrons 2: the states i in [1, s_L - 2] with o_L + i > 0
+ *            (pes.cpp:206-208); electrons 1: all zero (pes -e 1 prints no
+ *            excited population).
+ *   NORM     all ones (pes.cpp:138-142, 250-254).
+ * E[L] holds the s[L] energies of block L as tdse reads them
+ * (tdrd::readEnergies).  W: MPCI_OBS_ROWS x N doubles.  Host code only. */
+int mpci_obs_weights(int nb, const int *s, const double *const *E,
+                     int electrons, double threshold, double *W);
+/* The same observables of every pulse's current state, evaluated on the
+ * device from the system's energies: out[o * P + p] for o < MPCI_OBS_ROWS.
+ * The weights stay on the device between calls of the same kind. */
+int mpci_prop_observables(mpci_prop *prop, int electrons, double threshold,
                           double *out);
 
 /* Launch the 13-stage step as one captured hipGraph (1, default) or as

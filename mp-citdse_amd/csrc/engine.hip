@@ -61,6 +61,11 @@ struct DevBuf {
     HIPCHK(hipMalloc(&p, sizeof(T) * count));
     return MPCI_OK;
   }
+  void swap(DevBuf &o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
+  size_t bytes() const { return sizeof(T) * n; }
 };
 
 #define TRY(x)                  \
@@ -99,6 +104,8 @@ struct mpci_system {
   DevBuf<long long> drt_off;
   DevBuf<int> drt_ld_d;
   std::vector<std::vector<double>> hostD;
+  // the energies as given, for the observable weights (pes's masks)
+  std::vector<std::vector<double>> hostE;
   // autotuned GEMM tile per batch size P: every propagator of this system
   // with the same P uses the same tiles, hence the same summation order
   struct Tuned {
@@ -144,6 +151,10 @@ struct mpci_prop {
   double gemm_ms = 0.0;  // autotuned time of one coupling evaluation
   std::vector<GemmJob> gjobs;
   DevBuf<GemmJob> dgjobs;
+  // pes observable weights on the device (mpci_prop_observables)
+  DevBuf<double> obsW, obsOut;
+  int obs_electrons = 0;
+  double obs_threshold = 0.0;
   bool instrument = false;
   std::vector<hipEvent_t> ev;
   int ev_base = 0;
@@ -358,6 +369,8 @@ extern "C" int mpci_system_create(int nb, const int *s, char gauge,
   sys->hostD.resize(nb - 1);
   for (int L = 0; L + 1 < nb; ++L)
     sys->hostD[L].assign(D[L], D[L] + (size_t)s[L] * s[L + 1]);
+  sys->hostE.resize(nb);
+  for (int L = 0; L < nb; ++L) sys->hostE[L].assign(E[L], E[L] + s[L]);
   *out = sys;
   return MPCI_OK;
 }
@@ -855,14 +868,24 @@ extern "C" int mpci_prop_get_state(mpci_prop *pr, int p, double *ct) {
   return MPCI_OK;
 }
 
+static void drop_graph(mpci_prop *pr);
+
+// The envelope store only grows.  A captured step holds its address in the
+// stage kernels' arguments, so when it has to move, the graph is dropped
+// (after the stream has drained) and recaptured at the next run; samples
+// rewritten in place need no recapture.
 static int upload_pulses(mpci_prop *pr) {
   size_t tot = 0;
   for (int p = 0; p < pr->P; ++p) {
     pr->hpulses[p].env_off = (long long)tot;
     tot += pr->henv[p].size();
   }
-  if (tot > 0) {
+  if (tot > pr->env.n) {
+    HIPCHK(hipStreamSynchronize(pr->stream));
+    drop_graph(pr);
     TRY(pr->env.alloc(tot));
+  }
+  if (tot > 0) {
     std::vector<double> flat;
     flat.reserve(tot);
     for (auto &v : pr->henv) flat.insert(flat.end(), v.begin(), v.end());
@@ -1481,15 +1504,40 @@ extern "C" int mpci_prop_adjoint(mpci_prop *pr, const double *t0,
       return fail(MPCI_ERR_ARG, "grad_stride smaller than n_env");
   }
   const int K = ckpt_every;
-  const int ckcap = max_steps / K + 2;
   DevBuf<double2> ckpt, xs_store, xn_store, mu, lam, lamx, xfinal;
   DevBuf<double> nrm_store, tn_tab, hn_tab, gdev, red2, dotv, Wd;
-  TRY(ckpt.alloc((size_t)ckcap * NP));
+  // device bytes held by the adjoint's own buffers, and their peak
+  long long held = 0, peak = 0;
+  auto hold = [&](long long bytes) {
+    held += bytes;
+    peak = std::max(peak, held);
+  };
+  // Checkpoint slot j holds the state after j K accepted steps.  The store
+  // grows with the forward pass, which runs in chunks of at most logcap
+  // attempts: before each chunk it is made large enough for every step the
+  // chunk could accept, so it is sized by the run, not by max_steps.
+  int ckcap = 0;
+  auto grow_ckpt = [&](int need) -> int {
+    if (need <= ckcap) return MPCI_OK;
+    const int cap = std::max(need, ckcap + ckcap / 2);
+    DevBuf<double2> bigger;
+    TRY(bigger.alloc((size_t)cap * NP));
+    hold((long long)bigger.bytes());
+    if (ckcap > 0)
+      HIPCHK(hipMemcpyAsync(bigger.p, ckpt.p, ckpt.bytes(),
+                            hipMemcpyDeviceToDevice, pr->stream));
+    HIPCHK(hipStreamSynchronize(pr->stream));
+    held -= (long long)ckpt.bytes();
+    ckpt.swap(bigger);
+    ckcap = cap;
+    pr->store.ckpt = ckpt.p;
+    pr->store.ckpt_cap = ckcap;
+    return MPCI_OK;
+  };
   TRY(xfinal.alloc(NP));
   TRY(Wd.alloc(N));
+  hold((long long)(xfinal.bytes() + Wd.bytes()));
   TRY(h2d(pr, Wd.p, W, sizeof(double) * N));
-  HIPCHK(hipMemcpyAsync(ckpt.p, pr->X.p, sizeof(double2) * NP,
-                        hipMemcpyDeviceToDevice, pr->stream));
   const bool graph_was = pr->use_graph;
   pr->use_graph = false;
   drop_graph(pr);
@@ -1507,13 +1555,26 @@ extern "C" int mpci_prop_adjoint(mpci_prop *pr, const double *t0,
   HIPCHK(hipStreamSynchronize(pr->stream));
   auto tf = std::chrono::steady_clock::now();
   pr->store = StepStore{};
-  pr->store.ckpt = ckpt.p;
   pr->store.ckpt_every = K;
-  pr->store.ckpt_cap = ckcap;
+  TRY(grow_ckpt(2));
+  HIPCHK(hipMemcpyAsync(ckpt.p, pr->X.p, sizeof(double2) * NP,
+                        hipMemcpyDeviceToDevice, pr->stream));
   std::vector<Ctl> c;
   std::vector<std::vector<LogEnt>> acc;
-  TRY(run_adaptive(pr, t0, t1, h0, atol, rtol, max_steps * 8 + 64, 0, c,
-                   acc));
+  TRY(setup_adaptive(pr, t0, t1, h0, atol, rtol, max_steps * 8 + 64, 0, c,
+                     acc));
+  TRY(write_ctl(pr, c));
+  for (int n; (n = plan_chunk(pr, c)) > 0;) {
+    int done_max = 0;
+    for (const Ctl &x : c) done_max = std::max(done_max, x.accepted);
+    if (done_max > max_steps)
+      return fail(MPCI_ERR_LIMIT, "adjoint forward took more than max_steps "
+                                  "accepted steps");
+    TRY(grow_ckpt((done_max + n) / K + 2));
+    TRY(run_attempts(pr, n));
+    TRY(read_ctl(pr, c));
+    TRY(drain_log(pr, c, acc));
+  }
   HIPCHK(hipStreamSynchronize(pr->stream));
   const double fwd_ms = ms_since(tf);
   std::vector<int> nst(P);
@@ -1568,6 +1629,10 @@ extern "C" int mpci_prop_adjoint(mpci_prop *pr, const double *t0,
   TRY(tn_tab.alloc((size_t)K * P));
   TRY(hn_tab.alloc((size_t)K * P));
   TRY(gdev.alloc((size_t)K * kStages * P));
+  hold((long long)(lam.bytes() + lamx.bytes() + red2.bytes() + dotv.bytes() +
+                   xs_store.bytes() + xn_store.bytes() + nrm_store.bytes() +
+                   mu.bytes() + tn_tab.bytes() + hn_tab.bytes() +
+                   gdev.bytes()));
   for (int p = 0; p < P; ++p)
     for (int k = 0; k < grad_stride; ++k) grad[(size_t)p * grad_stride + k] = 0;
 
@@ -1679,6 +1744,8 @@ extern "C" int mpci_prop_adjoint(mpci_prop *pr, const double *t0,
     stats->segments = nseg;
     stats->fwd_ms = fwd_ms;
     stats->bwd_ms = bwd_ms;
+    stats->peak_bytes = peak;
+    stats->ckpt_slots = ckcap;
   }
   return MPCI_OK;
 }
@@ -1697,6 +1764,95 @@ extern "C" int mpci_prop_populations(mpci_prop *pr, int nobs, const double *W,
   HIPCHK(hipStreamSynchronize(pr->stream));
   TRY(d2h(pr, out, dout.p, sizeof(double) * nobs * pr->P));
   return MPCI_OK;
+}
+
+// ---------------------------------------------------------------------
+// pes observables (src/pes/pes.cpp): every scalar that pes prints is a sum
+// of |c_k|^2 over a set of states, so each is one row of weights, and the
+// device evaluates all rows for all pulses with one k_weighted_pop launch.
+// ---------------------------------------------------------------------
+extern "C" int mpci_obs_weights(int nb, const int *s, const double *const *E,
+                                int electrons, double threshold, double *W) {
+  if (nb < 1 || nb > kMaxBlocks || !s || !E || !W ||
+      (electrons != 1 && electrons != 2))
+    return fail(MPCI_ERR_ARG, "mpci_obs_weights: need 1 <= nb <= 32, s, E, "
+                              "W and electrons 1 or 2");
+  long long N = 0;
+  for (int L = 0; L < nb; ++L) {
+    if (s[L] < 1 || !E[L])
+      return fail(MPCI_ERR_ARG, "mpci_obs_weights: empty block");
+    N += s[L];
+  }
+  if (electrons == 1 && (nb < 2 || s[0] > s[1]))
+    return fail(MPCI_ERR_ARG,
+                "mpci_obs_weights: the one-electron yield bins onto the l = 1 "
+                "energies and needs at least two blocks with s[0] <= s[1] "
+                "(pes.cpp:97-100 reads l = 1 energies up to index s[0] - 1)");
+  std::fill(W, W + MPCI_OBS_ROWS * N, 0.0);
+  double *ground = W + MPCI_OBS_GROUND * N, *yield = W + MPCI_OBS_YIELD * N,
+         *excited = W + MPCI_OBS_EXCITED * N, *norm = W + MPCI_OBS_NORM * N;
+  ground[0] = 1.0;
+  std::fill(norm, norm + N, 1.0);
+  long long off = 0;
+  for (int L = 0; L < nb; ++L) {
+    const double *e = E[L];
+    if (electrons == 2) {
+      // interior states of each L above the one-electron threshold
+      // (pes.cpp:199-208): the first and last state of a block take part
+      // in neither sum
+      for (int i = 1; i + 1 < s[L]; ++i) {
+        if (e[i] - threshold > 0.0) yield[off + i] = 1.0;
+        if (off + i > 0) excited[off + i] = 1.0;
+      }
+    } else {
+      // each state i >= 1 of block l goes to the first bin k in
+      // [1, s[0] - 2] of the l = 1 spectrum that holds it, each bin reaching
+      // halfway to its neighbours (open at both ends); the bins of positive
+      // energy make the yield (pes.cpp:96-110)
+      const double *bin = E[1];
+      for (int i = 1; i < s[L]; ++i) {
+        for (int k = 1; k + 1 < s[0]; ++k) {
+          const double below = std::fabs(bin[k] - bin[k - 1]);
+          const double above = std::fabs(bin[k + 1] - bin[k]);
+          if (e[i] > bin[k] - 0.5 * below && e[i] < bin[k] + 0.5 * above) {
+            if (bin[k] > 0) yield[off + i] = 1.0;
+            break;
+          }
+        }
+      }
+    }
+    off += s[L];
+  }
+  return MPCI_OK;
+}
+
+extern "C" int mpci_prop_observables(mpci_prop *pr, int electrons,
+                                     double threshold, double *out) {
+  if (!pr || !out || (electrons != 1 && electrons != 2))
+    return fail(MPCI_ERR_ARG, "mpci_prop_observables: bad argument");
+  mpci_system *sys = pr->sys;
+  const int N = sys->N;
+  // the weights stay on the device for repeated calls with the same kind
+  if (!pr->obsW.p || pr->obs_electrons != electrons ||
+      !(pr->obs_threshold == threshold)) {
+    std::vector<const double *> E(sys->nb);
+    for (int L = 0; L < sys->nb; ++L) E[L] = sys->hostE[L].data();
+    std::vector<double> W((size_t)MPCI_OBS_ROWS * N);
+    TRY(mpci_obs_weights(sys->nb, sys->s.data(), E.data(), electrons,
+                         threshold, W.data()));
+    if (!pr->obsW.p) {
+      TRY(pr->obsW.alloc(W.size()));
+      TRY(pr->obsOut.alloc((size_t)MPCI_OBS_ROWS * pr->P));
+    }
+    TRY(h2d(pr, pr->obsW.p, W.data(), sizeof(double) * W.size()));
+    pr->obs_electrons = electrons;
+    pr->obs_threshold = threshold;
+  }
+  hipLaunchKernelGGL(k_weighted_pop, dim3(pr->P, MPCI_OBS_ROWS), dim3(256), 0,
+                     pr->stream, pr->X.p, pr->obsW.p, N, pr->P,
+                     pr->obsOut.p);
+  HIPCHK(hipGetLastError());
+  return d2h(pr, out, pr->obsOut.p, sizeof(double) * MPCI_OBS_ROWS * pr->P);
 }
 
 extern "C" int mpci_prop_set_graph(mpci_prop *pr, int enable) {

@@ -19,54 +19,58 @@ int readConfig(std::string file, std::string &pot, std::string set_base,
                std::vector<int> &state_sz, double &timestep,
                std::string &shape, double &w, double &Io, double &cepd,
                int &cycles, Extensions *ext) {
-  mpci::YamlDoc settings = mpci::YamlDoc::load_file(file);
-  std::cout << "Global Settings:" << std::endl;
-  pot = settings.str("Global_Settings", "potential");
-  std::cout << "  Core Potential:                       " << pot << std::endl;
-  L_max = settings.as_int(set_base, option);
-  std::cout << "  max l/L:                              " << L_max << std::endl;
-  gauge = settings.as_char("Global_Settings", "gauge");
-  std::cout << "  Gauge type ('l' length/'v' velocity): " << gauge << std::endl;
+  const mpci::YamlDoc doc = mpci::YamlDoc::load_file(file);
+  const char *glob = "Global_Settings", *prop = "Propagator_Settings",
+             *field = "Field_Parameters";
+  // The stdout echo is part of the drop-in contract: each value is printed
+  // as soon as it is read, under its section heading, with the label
+  // padded to the value column of td_read.cpp:10-45.
+  auto section = [](const char *title) { std::cout << title << std::endl; };
+  auto show = [](const char *label, const auto &value) {
+    std::cout << "  " << label << value << std::endl;
+  };
 
-  std::cout << "Propagator Settings:" << std::endl;
-  std::vector<int> loc_sz =
-      settings.as_int_list("Propagator_Settings", "states_in_l");
-  if (static_cast<int>(loc_sz.size()) != L_max + 1)
+  section("Global Settings:");
+  show("Core Potential:                       ",
+       pot = doc.str(glob, "potential"));
+  show("max l/L:                              ",
+       L_max = doc.as_int(set_base, option));
+  show("Gauge type ('l' length/'v' velocity): ",
+       gauge = doc.as_char(glob, "gauge"));
+
+  section("Propagator Settings:");
+  const std::vector<int> sizes = doc.as_int_list(prop, "states_in_l");
+  if (static_cast<int>(sizes.size()) != L_max + 1)
     throw std::runtime_error("Propagator_Settings.states_in_l must have "
                              "L_max + 1 entries");
-  std::cout << "  states per l/L:          ";
-  for (auto i = 0; i < static_cast<int>(loc_sz.size()); ++i) {
-    std::cout << loc_sz[i] << " ";
-    state_sz.push_back(loc_sz[i]);
-  }
-  std::cout << std::endl;
-  timestep = settings.as_double("Propagator_Settings", "dt");
-  std::cout << "  timestep dt:                          " << timestep
-            << std::endl;
+  state_sz.insert(state_sz.end(), sizes.begin(), sizes.end());
+  std::string size_list;
+  for (const int n : sizes) size_list += std::to_string(n) + ' ';
+  show("states per l/L:          ", size_list);
+  show("timestep dt:                          ",
+       timestep = doc.as_double(prop, "dt"));
 
-  std::cout << "Field Parameters:" << std::endl;
-  shape = settings.str("Field_Parameters", "shape");
-  std::cout << "  envelope shape:                       " << shape << std::endl;
-  w = settings.as_double("Field_Parameters", "w");
-  std::cout << "  photon energy:                        " << w << std::endl;
-  Io = settings.as_double("Field_Parameters", "Io");
-  std::cout << "  Peak intensity:                       " << Io << std::endl;
-  cepd = settings.as_double("Field_Parameters", "cepd");
-  std::cout << "  phase shift:                          " << cepd << std::endl;
-  cycles = settings.as_int("Field_Parameters", "cycles");
-  std::cout << "  number of cycles:                     " << cycles
-            << std::endl;
+  section("Field Parameters:");
+  show("envelope shape:                       ",
+       shape = doc.str(field, "shape"));
+  show("photon energy:                        ", w = doc.as_double(field, "w"));
+  show("Peak intensity:                       ",
+       Io = doc.as_double(field, "Io"));
+  show("phase shift:                          ",
+       cepd = doc.as_double(field, "cepd"));
+  show("number of cycles:                     ",
+       cycles = doc.as_int(field, "cycles"));
 
   if (ext) {
-    const char *ps = "Propagator_Settings";
-    if (settings.has(ps, "mode")) ext->mode = settings.str(ps, "mode");
-    if (settings.has(ps, "tol")) {
-      ext->atol = ext->rtol = settings.as_double(ps, "tol");
-    }
-    if (settings.has(ps, "atol")) ext->atol = settings.as_double(ps, "atol");
-    if (settings.has(ps, "rtol")) ext->rtol = settings.as_double(ps, "rtol");
-    if (settings.has(ps, "full_precision"))
-      ext->full_precision = settings.as_bool(ps, "full_precision");
+    // optional keys of this build, absent from the reference's inputs
+    if (doc.has(prop, "mode")) ext->mode = doc.str(prop, "mode");
+    if (doc.has(prop, "tol")) ext->atol = ext->rtol = doc.as_double(prop, "tol");
+    if (doc.has(prop, "atol")) ext->atol = doc.as_double(prop, "atol");
+    if (doc.has(prop, "rtol")) ext->rtol = doc.as_double(prop, "rtol");
+    if (doc.has(prop, "full_precision"))
+      ext->full_precision = doc.as_bool(prop, "full_precision");
+    if (doc.has(prop, "observables"))
+      ext->observables = doc.as_bool(prop, "observables");
     if (ext->mode != "fixed" && ext->mode != "adaptive")
       throw std::runtime_error("Propagator_Settings.mode must be \"fixed\" "
                                "or \"adaptive\"");
@@ -129,14 +133,27 @@ int readInitCt(std::string file, int ct_sz, std::vector<double> &ct) {
   return 0;
 }
 
+// The time is the number that starts at the first digit of the file name
+// (tdse_main.cpp:153-157), which for the reference's <pot>_ct_<t>.dat files
+// is <t>.  The full-precision side files <pot>_ct17_<t>.dat carry a digit in
+// their tag, so for them the number is read after the tag instead.
 double timeFromCtName(const std::string &path) {
-  std::string fln = path;
-  size_t slash = fln.find_last_of('/');
-  if (slash != std::string::npos) fln = fln.substr(slash + 1);
-  size_t t_start = fln.find_first_of("0123456789");
-  if (t_start == std::string::npos)
-    throw std::runtime_error("no start time in file name " + fln);
-  return std::strtod(fln.c_str() + t_start, nullptr);
+  static const std::string kFullTag = "_ct17_";
+  const size_t slash = path.find_last_of('/');
+  const std::string name =
+      slash == std::string::npos ? path : path.substr(slash + 1);
+  const size_t tag = name.rfind(kFullTag);
+  const size_t from = tag != std::string::npos
+                          ? tag + kFullTag.size()
+                          : name.find_first_of("0123456789");
+  if (from == std::string::npos || from >= name.size())
+    throw std::runtime_error("no start time in file name " + name);
+  const char *begin = name.c_str() + from;
+  char *end = nullptr;
+  const double t = std::strtod(begin, &end);
+  if (end == begin)
+    throw std::runtime_error("no start time in file name " + name);
+  return t;
 }
 
 }  // namespace tdrd

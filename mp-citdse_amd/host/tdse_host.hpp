@@ -22,6 +22,11 @@ struct Extensions {
   double atol = 1e-10;
   double rtol = 1e-10;
   bool full_precision = false;  // extra <pot>_ct17_<t>.dat files
+  // <pot>_obs.dat: pes's ground population, yield, excited population and
+  // norm (src/pes/pes.cpp:138-146, 199-213) at every coefficient-file time
+  bool observables = false;
+  int obs_electrons = 2;       // which of pes's definitions (1 or 2)
+  double obs_threshold = 0.0;  // two electrons: dat/<pot>0.h5:/En[0]
 };
 
 // td_read.cpp:3-48 (echoes the settings to stdout exactly as the reference).

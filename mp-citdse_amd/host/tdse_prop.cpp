@@ -50,6 +50,33 @@ int make_engine(Engine &eng, char gauge, int L_max,
                 "mpci_prop_set_state");
 }
 
+// <pot>_obs.dat (extension): pes's scalars of the device state at every
+// coefficient-file time, evaluated on the GPU (mpci_prop_observables).
+class ObsLog {
+ public:
+  ObsLog(const std::string &output, const tdrd::Extensions &ext) : ext_(ext) {
+    if (!ext.observables) return;
+    f_.open(output + "_obs.dat", std::ios::out);
+    f_ << "#time (a.u.), ground state pop, yield, excited population, "
+          "norm\n";
+  }
+  int record(mpci_prop *prop, double t) {
+    if (!ext_.observables) return MPCI_OK;
+    double v[MPCI_OBS_ROWS];
+    const int rc = mpci_prop_observables(prop, ext_.obs_electrons,
+                                         ext_.obs_threshold, v);
+    if (rc != MPCI_OK) return report(rc, "mpci_prop_observables");
+    f_ << std::setprecision(16) << t;
+    for (double x : v) f_ << " " << x;
+    f_ << "\n";
+    return MPCI_OK;
+  }
+
+ private:
+  const tdrd::Extensions &ext_;
+  std::fstream f_;
+};
+
 double popOf(const std::vector<double> &ct, int k) {
   return std::norm(std::complex<double>(ct[2 * k], ct[2 * k + 1]));
 }
@@ -66,6 +93,8 @@ int prop_fixed(char gauge, std::string output, int L_max, double t, double dt,
   Engine eng;
   int rc = make_engine(eng, gauge, L_max, state_sz, eig, dipoles, ct);
   if (rc != MPCI_OK) return rc;
+  ObsLog obs(output, ext);
+  if ((rc = obs.record(eng.prop, t)) != MPCI_OK) return rc;
 
   std::fstream field_fl, f_pop;
   field_fl.open(output + "_field.dat", std::ios::out);
@@ -108,12 +137,14 @@ int prop_fixed(char gauge, std::string output, int L_max, double t, double dt,
       std::cout << F[n - 1] << "\n";
       tdse::writeCt(output, t, ct, ct_sz);
       if (ext.full_precision) tdse::writeCt(output, t, ct, ct_sz, 17);
+      if ((rc = obs.record(eng.prop, t)) != MPCI_OK) return rc;
     }
   }
   rc = mpci_prop_get_state(eng.prop, 0, ct.data());
   if (rc != MPCI_OK) return report(rc, "mpci_prop_get_state");
   tdse::writeCt(output, t, ct, ct_sz);
   if (ext.full_precision) tdse::writeCt(output, t, ct, ct_sz, 17);
+  if ((rc = obs.record(eng.prop, t)) != MPCI_OK) return rc;
   field_fl.close();
   f_pop.close();
   return 0;
@@ -165,6 +196,8 @@ int propAdaptive(std::string output, char gauge, int L_max, double t,
   if (rc != MPCI_OK) return rc;
   rc = mpci_prop_set_pulse(eng.prop, 0, &pars);
   if (rc != MPCI_OK) return report(rc, "mpci_prop_set_pulse");
+  ObsLog obs(output, ext);
+  if ((rc = obs.record(eng.prop, t)) != MPCI_OK) return rc;
 
   std::fstream field_fl, f_pop;
   field_fl.open(output + "_field.dat", std::ios::out);
@@ -201,6 +234,7 @@ int propAdaptive(std::string output, char gauge, int L_max, double t,
     std::cout << mpci_pulse_eval(&pars, t) << "\n";
     writeCt(output, t, ct, ct_sz);
     if (ext.full_precision) writeCt(output, t, ct, ct_sz, 17);
+    if ((rc = obs.record(eng.prop, t)) != MPCI_OK) return rc;
   }
   field_fl.close();
   f_pop.close();

@@ -22,6 +22,10 @@ MPCI_ERR_NOTSUP = -5
 
 PULSE_SINE2, PULSE_GAUSS, PULSE_PWL = 0, 1, 2
 
+# rows of the observable weights (MPCI_OBS_*)
+OBS_GROUND, OBS_YIELD, OBS_EXCITED, OBS_NORM = 0, 1, 2, 3
+OBS_ROWS = 4
+
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _vp = C.c_void_p
@@ -42,7 +46,8 @@ class MpciStats(C.Structure):
 
 class MpciAdjointStats(C.Structure):
     _fields_ = [("steps_max", C.c_int), ("segments", C.c_int),
-                ("fwd_ms", C.c_double), ("bwd_ms", C.c_double)]
+                ("fwd_ms", C.c_double), ("bwd_ms", C.c_double),
+                ("peak_bytes", C.c_longlong), ("ckpt_slots", C.c_int)]
 
 
 class MpciError(RuntimeError):
@@ -89,6 +94,9 @@ _SIGS = [
                                       C.POINTER(C.POINTER(MpciStats))]),
     ("mpci_prop_schedule", C.c_int, [_vp, _dp, C.c_int, _dp]),
     ("mpci_prop_populations", C.c_int, [_vp, C.c_int, _dp, _dp]),
+    ("mpci_obs_weights", C.c_int, [C.c_int, _ip, C.POINTER(_dp), C.c_int,
+                                   C.c_double, _dp]),
+    ("mpci_prop_observables", C.c_int, [_vp, C.c_int, C.c_double, _dp]),
     ("mpci_prop_adjoint", C.c_int, [_vp, _dp, _dp, _dp, C.c_double,
                                     C.c_double, C.c_int, _dp, C.c_int, _dp,
                                     _dp, C.c_int, _ip, _dp, C.c_int,

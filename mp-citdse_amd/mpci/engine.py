@@ -219,6 +219,16 @@ class Propagator:
                                                     dptr(W), dptr(out)))
         return out
 
+    def observables(self, electrons: int, threshold: float = 0.0):
+        """pes's scalars for every pulse's current state, on the device:
+        an (OBS_ROWS, P) array with rows ground population, yield, excited
+        population and norm (src/pes/pes.cpp:92-110, 138-146, 199-213)."""
+        out = np.empty((_lib.OBS_ROWS, self.P))
+        check(_lib.tdse_lib().mpci_prop_observables(self.h, int(electrons),
+                                                    float(threshold),
+                                                    dptr(out)))
+        return out
+
     def set_graph(self, on: bool):
         check(_lib.tdse_lib().mpci_prop_set_graph(self.h, int(on)))
 
@@ -298,6 +308,20 @@ def multi_adaptive(props, t0, t1, h0, atol, rtol, max_attempts=10_000_000,
     if not (allow_limit and rc == _lib.MPCI_ERR_LIMIT):
         check(rc)
     return [list(s) for s in st]
+
+
+def obs_weights(E, electrons: int, threshold: float = 0.0) -> np.ndarray:
+    """Per-state weights of pes's scalars (mpci_obs_weights): an
+    (OBS_ROWS, N) array; observable o of a state c is W[o] @ |c|^2.  E: the
+    energies of each block as tdse reads them.  Host code, no GPU needed."""
+    E = [np.ascontiguousarray(e, dtype=np.float64) for e in E]
+    s = np.array([len(e) for e in E], dtype=np.int32)
+    Ep = (_lib._dp * len(E))(*[dptr(e) for e in E])
+    W = np.empty((_lib.OBS_ROWS, int(s.sum())))
+    check(_lib.tdse_lib().mpci_obs_weights(len(E), s.ctypes.data_as(_lib._ip),
+                                           Ep, int(electrons),
+                                           float(threshold), dptr(W)))
+    return W
 
 
 def device_count() -> int:

@@ -161,10 +161,18 @@ class System:
         return x
 
     def observables(self, x, thr):
+        """(ground pop, 2e yield, excitation, norm), pes.cpp:151-262."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         out = np.empty(4)
         lib().or_observables(C.byref(self.c), _ptr(x), C.c_double(thr),
                              _ptr(out))
+        return out
+
+    def observables_1e(self, x):
+        """(ground pop, 1e binned yield, 0, norm), pes.cpp:48-149."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty(4)
+        lib().or_observables_1e(C.byref(self.c), _ptr(x), _ptr(out))
         return out
 
 

@@ -515,3 +515,34 @@ void or_observables(const or_sys *sys, const double *x, double thr,
   out[2] = exc;
   out[3] = nrm;
 }
+
+void or_observables_1e(const or_sys *sys, const double *x, double *out) {
+  /* pes.cpp:90-110 (yield), 138-146 (ground population, norm).  PES_En is
+   * the l = 1 block of energies; state i >= 1 of block l counts toward the
+   * yield through the first bin k in [1, s_0 - 2] that holds its energy, if
+   * PES_En[k] > 0. */
+  const double *pes_en = sys->E[1];
+  double yield = 0.0, nrm = 0.0;
+  int off = 0;
+  for (int l = 0; l < sys->nb; ++l) {
+    for (int i = 1; i < sys->s[l]; ++i) {
+      for (int k = 1; k < sys->s[0] - 1; ++k) {
+        double ndiff = fabs(pes_en[k] - pes_en[k - 1]);
+        double pdiff = fabs(pes_en[k + 1] - pes_en[k]);
+        double en = sys->E[l][i];
+        if (en > pes_en[k] - 0.5 * ndiff && en < pes_en[k] + 0.5 * pdiff) {
+          const double *c = x + 2 * (off + i);
+          yield += (pes_en[k] > 0) * (c[0] * c[0] + c[1] * c[1]);
+          break;
+        }
+      }
+    }
+    off += sys->s[l];
+  }
+  for (int i = 0; i < off; ++i)
+    nrm += x[2 * i] * x[2 * i] + x[2 * i + 1] * x[2 * i + 1];
+  out[0] = x[0] * x[0] + x[1] * x[1];
+  out[1] = yield;
+  out[2] = 0.0;
+  out[3] = nrm;
+}

@@ -28,11 +28,14 @@
  *                      given list of step sizes (used for finite differences).
  *   or_observables     src/pes/pes.cpp:151-262 (2e yield, excitation,
  *                      ground population, norm).
+ *   or_observables_1e  src/pes/pes.cpp:48-149 (1e yield binned onto the
+ *                      l = 1 energies, ground population, norm).
  *
  * Parity pins (tests/test_oracle_*.py): exact-rational order conditions of
  * the tableau; the RKF78 stability polynomial R(z) on diagonal and 2x2
  * constant systems; dense numpy -iH on tiny random blocks; closed forms of
- * the pulses; hand-computed observables.  boost's exact summation order and
+ * the pulses; hand-computed observables (2e masks and a 1e binning case
+ * with energies on and between the bin edges).  boost's exact summation order and
  * BLAS's reduction order are parity unpinned (not needed at 1e-12).
  */
 #ifndef MPCI_TDSE_ORACLE_H_
@@ -137,6 +140,11 @@ void or_prop_schedule(const or_sys *sys, const or_pulse *p, double t0,
  * out[0]=ground pop, out[1]=2e yield, out[2]=excitation, out[3]=norm. */
 void or_observables(const or_sys *sys, const double *x, double thr,
                     double *out);
+/* One-electron observables (src/pes/pes.cpp:48-149): out[0]=ground pop,
+ * out[1]=1e yield, out[2]=0 (pes -e 1 prints no excitation), out[3]=norm.
+ * Needs nb >= 2 and s[0] <= s[1] (the bins read l = 1 energies up to
+ * index s[0] - 1). */
+void or_observables_1e(const or_sys *sys, const double *x, double *out);
 
 #ifdef __cplusplus
 }

@@ -164,3 +164,25 @@ def test_torch_autograd_wrapper(mpci):
     assert np.allclose(J.detach().numpy(), Jr, rtol=0, atol=1e-14)
     assert np.allclose(env.grad.numpy()[0], gr[0], rtol=1e-12, atol=0)
     assert np.allclose(env.grad.numpy()[1], -2.0 * gr[1], rtol=1e-12, atol=0)
+
+
+def test_checkpoint_store_sized_by_the_run(mpci):
+    """The checkpoint store grows with the forward pass: its slots follow
+    the accepted steps taken, not max_steps (whose default would ask for
+    100000 / K slots)."""
+    from mpci import synth
+    E, D = synth.system_arrays([30, 24, 20], seed=19)
+    g = mpci.System(E, D, gauge="v")
+    prop = mpci.Propagator(g)
+    w, T, K = 0.7, 30.0, 8
+    prop.set_state(synth.ground_state(g.N))
+    prop.set_pulse(mpci.Pulse.pwl(synth.uniform(9, 0, 16, 0.0, 0.04), w, T))
+    W = np.zeros(g.N)
+    W[0] = 1.0
+    J, grad, nst, _, st = prop.adjoint(0.0, T, 0.05, W, ckpt_every=K,
+                                       max_steps=100000)
+    assert st.segments == math.ceil(nst[0] / K)
+    assert nst[0] // K + 2 <= st.ckpt_slots < 100000 // K
+    # the slots a chunk can fill, at most one chunk of attempts ahead
+    assert st.ckpt_slots <= (nst[0] + 4096) // K + 2
+    assert st.peak_bytes >= st.ckpt_slots * g.N * 16

@@ -296,3 +296,87 @@ def test_cli_golden_h_input(tmp_path, oracle):
     assert np.abs(pop[1:251, 1] - popr).max() < 1e-12
     xf = read_state(out / ("h_ct_%f.dat" % t), o.N)
     assert abs(np.dot(xf, xf) - 1.0) < 1e-4
+
+
+def test_cli_restart_full_precision(tmp_path, oracle):
+    """Restart from a 17-digit side file (<pot>_ct17_<t>.dat): the start
+    time is read after the tag (not the 17 of the tag), the run continues
+    from the full-precision state, and its final state matches the oracle
+    started from the same state within 1e-12."""
+    full = states = [30, 30, 30]
+    subprocess.run([os.path.join(BIN, "tdse_gen"), "-e", "2", "-p", "he",
+                    "-s", "30,30,30", "-g", "v"], cwd=tmp_path, check=True)
+    (tmp_path / "He.yaml").write_text(yaml_2e(states, gauge="v"))
+    run(["-e", "2", "-f", "He.yaml", "-o", "out"], tmp_path)
+    names = sorted(f for f in os.listdir(tmp_path / "out")
+                   if f.startswith("he_ct17_"))
+    ct = names[3]
+    t_start = float(ct[len("he_ct17_"):-4])
+    assert 0.0 < t_start < 17.0
+    r = run(["-e", "2", "-f", "He.yaml", "-o", "out2", "-s", f"out/{ct}"],
+            tmp_path)
+    assert f"Start time (a.u.): {t_start:g}" in r.stdout
+    E, D = leading(full, states)
+    o = oracle.System(E, D, gauge="v")
+    w = 15 * EV
+    tau = 2 * 2.0 * math.pi / w
+    steps = int((tau - t_start) / 0.1)
+    x0 = read_state(tmp_path / "out" / ct, o.N)  # 17-digit restart values
+    pulse = oracle.Pulse.sine(1.0e14 * IO, w, 2, gauge="v")
+    xr, tr, popr, _ = o.prop_fixed(pulse, t_start, 0.1, steps, x0)
+    x = read_state(tmp_path / "out2" / ("he_ct17_%f.dat" % tr), o.N)
+    assert np.abs(x - xr).max() < 1e-12
+    pop = np.loadtxt(tmp_path / "out2" / "he_pop10.dat", comments="#")
+    assert pop.shape == (steps + 1, 2)
+    assert np.abs(pop[1:, 1] - popr).max() < 1e-12
+
+
+def _obs_lines(path):
+    lines = path.read_text().splitlines()
+    assert lines[0] == ("#time (a.u.), ground state pop, yield, excited "
+                        "population, norm")
+    return np.array([[float(v) for v in ln.split()] for ln in lines[1:]])
+
+
+@pytest.mark.parametrize("electrons", [2, 1])
+def test_cli_observables_side_file(tmp_path, oracle, electrons):
+    """Propagator_Settings.observables: true adds <pot>_obs.dat with pes's
+    ground population, yield, excited population and norm (16 digits) at
+    every coefficient-file time, evaluated on the device; the last line
+    matches the oracle's observables (or_observables with the threshold of
+    dat/<pot>0.h5:/En[0], or or_observables_1e) of the oracle's final
+    state within 1e-12."""
+    pot = "he" if electrons == 2 else "h"
+    full = states = [30, 36, 28]
+    subprocess.run([os.path.join(BIN, "tdse_gen"), "-e", str(electrons),
+                    "-p", pot, "-s", "30,36,28", "-g", "v"], cwd=tmp_path,
+                   check=True)
+    y = yaml_2e(states).replace("  full_precision: true\n",
+                                "  full_precision: true\n"
+                                "  observables: true\n")
+    y = y.replace('potential: "he"', f'potential: "{pot}"')
+    if electrons == 1:
+        y = y.replace("l_max: 7", "l_max: 2")
+    (tmp_path / "in.yaml").write_text(y)
+    run(["-e", str(electrons), "-f", "in.yaml", "-o", "out"], tmp_path)
+    out = tmp_path / "out"
+    obs = _obs_lines(out / f"{pot}_obs.dat")
+    cts = sorted(float(f[len(pot) + 4:-4]) for f in os.listdir(out)
+                 if f.startswith(f"{pot}_ct_"))
+    # one line per coefficient file (the final time appears twice when the
+    # last step also ends a tenth, as for the ct files written there)
+    assert sorted(set(np.round(obs[:, 0], 6))) == sorted(set(cts))
+    assert obs[0, 1] == 1.0 and obs[0, 4] == 1.0
+    E, D = leading(full, states)
+    o = oracle.System(E, D, gauge="v")
+    w = 15 * EV
+    tau = 2 * 2.0 * math.pi / w
+    steps = int(tau / 0.1)
+    pulse = oracle.Pulse.sine(1.0e14 * IO, w, 2, gauge="v")
+    x0 = np.zeros(2 * o.N)
+    x0[0] = 1.0
+    xr, _, _, _ = o.prop_fixed(pulse, 0.0, 0.1, steps, x0)
+    ref = (o.observables(xr, -2.0) if electrons == 2
+           else o.observables_1e(xr))
+    assert np.abs(obs[-1, 1:] - ref).max() < 1e-12
+    assert abs(obs[-1, 4] - 1.0) < 1e-12

@@ -307,3 +307,73 @@ def test_measurement_probes(mpci):
         assert e.value.code == -1
     with pytest.raises(mpci.MpciError):
         mpci.probe_mfma_f64(8)
+
+
+@pytest.mark.parametrize("electrons", [1, 2])
+def test_observables_match_oracle(mpci, oracle, electrons):
+    """pes's observables on the device (mpci_prop_observables) for three
+    pulses propagated adaptively, against the oracle's or_observables /
+    or_observables_1e (src/pes/pes.cpp:48-262): on the GPU's own states
+    within 1e-13, and on the oracle's propagated states within the
+    adaptive-mode bar."""
+    from mpci import synth
+    s = [30, 36, 28]
+    lo, hi = (-0.5, 2.0) if electrons == 1 else (-2.9, 5.0)
+    E, D = synth.system_arrays(s, seed=31 + electrons, e_lo=lo, e_hi=hi)
+    g = mpci.System(E, D, gauge="v")
+    o = oracle.System(E, D, gauge="v")
+    P, w, cyc, thr = 3, 0.7, 2, -2.0
+    amps = [0.02, 0.05, 0.08]
+    T = mpci.period(w, cyc)
+    x0 = synth.ground_state(g.N)
+    prop = mpci.Propagator(g, P=P)
+    for p in range(P):
+        prop.set_state(x0, p)
+        prop.set_pulse(mpci.Pulse.sine(amps[p] ** 2, w, cyc), p)
+    prop.prop_adaptive(0.0, T, 0.05, 1e-10, 1e-10)
+    got = prop.observables(electrons, thr)
+    assert got.shape == (mpci.OBS_ROWS, P)
+    for p in range(P):
+        xg = prop.get_state(p)
+        ref_g = (o.observables_1e(xg) if electrons == 1
+                 else o.observables(xg, thr))
+        assert np.abs(got[:, p] - ref_g).max() < 1e-13
+        xo, _, _ = o.prop_adaptive(oracle.Pulse.sine(amps[p] ** 2, w, cyc),
+                                   0.0, T, 0.05, 1e-10, 1e-10, x0)
+        ref = (o.observables_1e(xo) if electrons == 1
+               else o.observables(xo, thr))
+        assert np.abs(got[:, p] - ref).max() < 1e-10
+    # the yield grows with the field strength, the norm stays 1
+    assert got[mpci.OBS_YIELD, 0] < got[mpci.OBS_YIELD, 2]
+    assert np.abs(got[mpci.OBS_NORM] - 1.0).max() < 1e-12
+
+
+def test_pwl_envelope_change_between_graph_runs(mpci, oracle):
+    """set_pulse with a longer piecewise-linear envelope moves the device
+    envelope store, which a captured step refers to: the graph is recaptured
+    and graph runs stay bit-equal to eager runs (then a same-length envelope
+    is rewritten in place), each run matching the oracle."""
+    from mpci import synth
+    g, o = build(mpci, oracle, [40, 52, 33], "v", seed=23)
+    w, T = 0.7, 20.0
+    envs = [synth.uniform(201, 0, 6, 0.0, 0.05),
+            synth.uniform(202, 0, 40, 0.0, 0.05),
+            synth.uniform(203, 0, 40, 0.0, 0.05)]
+    x0 = synth.ground_state(g.N)
+    runs = {}
+    for graph in (True, False):
+        prop = mpci.Propagator(g)
+        prop.set_graph(graph)
+        out = []
+        for env in envs:
+            prop.set_state(x0)
+            prop.set_pulse(mpci.Pulse.pwl(env, w, T))
+            prop.prop_adaptive(0.0, T, 0.05, 1e-10, 1e-10)
+            out.append(prop.get_state())
+        runs[graph] = out
+        prop.close()
+    for k, env in enumerate(envs):
+        assert np.array_equal(runs[True][k], runs[False][k]), k
+        xo, _, _ = o.prop_adaptive(oracle.Pulse.pwl(env, w, T), 0.0, T, 0.05,
+                                   1e-10, 1e-10, x0)
+        assert np.abs(runs[True][k] - xo).max() < 1e-10

@@ -130,13 +130,29 @@ def test_ct_file_format_and_read_back(io, tmp_path):
                               _dp(np.empty(6))) == -1
 
 
-@pytest.mark.parametrize("name,t", [("he_ct_12.300000.dat", 12.3),
-                                     ("/x/y/he_ct_0.000000.dat", 0.0),
-                                     ("h_ct_125.600000.dat", 125.6)])
+@pytest.mark.parametrize("name,t", [
+    ("he_ct_12.300000.dat", 12.3),
+    ("/x/y/he_ct_0.000000.dat", 0.0),
+    ("h_ct_125.600000.dat", 125.6),
+    # full-precision side files: the time follows the _ct17_ tag
+    ("he_ct17_12.300000.dat", 12.3),
+    ("out/he_ct17_0.000000.dat", 0.0),
+    ("/a/b1/h_ct17_125.600000.dat", 125.6),
+    # plain names keep the reference's rule, the number at the first digit
+    # of the file name (tdse_main.cpp:153-157), even a digit of the name
+    ("h2_ct_5.000000.dat", 2.0),
+])
 def test_restart_time_from_name(io, name, t):
     v = C.c_double()
     assert io.mpci_io_time_from_name(name.encode(), C.byref(v)) == 0
     assert v.value == t
+
+
+@pytest.mark.parametrize("name", ["he_ct_.dat", "out/he_ct17_.dat"])
+def test_restart_time_missing(io, name):
+    v = C.c_double()
+    assert io.mpci_io_time_from_name(name.encode(), C.byref(v)) == -1
+    assert b"start time" in io.mpci_io_last_error()
 
 
 def _cli(args, cwd, stdin=None):
@@ -177,3 +193,18 @@ def test_cli_missing_inputs_fail_cleanly(tmp_path):
     (tmp_path / "He.yaml").write_text(YAML)
     r = _cli(["-e", "2", "-f", "He.yaml", "-o", "out"], tmp_path)
     assert r.returncode == 1 and "tdse:" in r.stderr
+
+
+def test_cli_option_checks(tmp_path):
+    """Usage errors end with the reference's messages and its exit status
+    (-1); -l is checked itself (the reference tests -n there,
+    tdse_main.cpp:59-64), and a missing -e is refused before anything is
+    written."""
+    (tmp_path / "He.yaml").write_text(YAML)
+    cases = [(["-e", "2", "-n", "0"], "Invalid primary quantum number -n"),
+             (["-e", "2", "-l", "-1"], "Invalid angular momentum number -l"),
+             (["-f", "He.yaml", "-o", "out"], "Invalid number of electrons")]
+    for args, msg in cases:
+        r = _cli(args, tmp_path)
+        assert r.returncode == 255 and msg in r.stdout, (args, r.stdout)
+    assert not (tmp_path / "out").exists()

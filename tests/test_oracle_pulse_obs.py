@@ -82,3 +82,38 @@ def test_observables_hand_case(oracle):
     assert abs(out[1] - (p[2] + p[5])) < 1e-16
     assert abs(out[2] - (p[1] + p[2] + p[5])) < 1e-16
     assert abs(out[3] - p.sum()) < 1e-15
+
+
+def _pops(c):
+    x = np.empty(2 * len(c))
+    x[0::2], x[1::2] = c.real, c.imag
+    return x, np.abs(c) ** 2
+
+
+def test_observables_1e_hand_binning(oracle):
+    """genPES1e's yield (pes.cpp:90-110), worked out by hand.  Bins are the
+    l = 1 energies k = 1 .. s_0 - 2 = 3 of [-0.5, -0.125, 0.125, 0.25, 0.75,
+    1.0], each reaching halfway to its neighbours, open at both ends:
+      k = 1: centre -0.125, (-0.3125, 0.0)    negative: no yield
+      k = 2: centre  0.125, (0.0, 0.1875)     yield
+      k = 3: centre  0.25,  (0.1875, 0.5)     yield
+    (with the neighbour spacings swapped, k = 3 would be (0.0, 0.3125); with
+    bins up to s_1 - 2, the state at 0.75 would count; with i from 0, the
+    first state of l = 2 would count).  Every value is a multiple of 1/64,
+    so the edges are exact."""
+    E = [np.array([-2.0, -0.25, 0.0, 0.4375, 1.0]),
+         np.array([-0.5, -0.125, 0.125, 0.25, 0.75, 1.0]),
+         np.array([0.0625, 0.15625, 0.1875])]
+    D = [np.zeros((5, 6)), np.zeros((6, 3))]
+    sys_ = oracle.System(E, D)
+    c = np.exp(1j * np.arange(14)) * np.sqrt(np.arange(1, 15) / 105.0)
+    x, p = _pops(c)
+    out = sys_.observables_1e(x)
+    # l = 0: -0.25 in bin 1 (negative), 0.0 on the edge of bins 1 and 2
+    #        (in neither), 0.4375 in bin 3, 1.0 in none;
+    # l = 1: -0.125 bin 1, 0.125 bin 2, 0.25 bin 3, 0.75 and 1.0 none;
+    # l = 2: 0.15625 bin 2, 0.1875 on the edge of bins 2 and 3
+    assert abs(out[1] - (p[3] + p[7] + p[8] + p[12])) < 1e-16
+    assert abs(out[0] - p[0]) < 1e-16
+    assert out[2] == 0.0
+    assert abs(out[3] - p.sum()) < 1e-15
