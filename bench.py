@@ -91,11 +91,24 @@ def run_gpu(args, mpci, synth, sysg):
     prop.set_instrument(False)
     if st_i[0].accepted != st[0].accepted:
         raise RuntimeError("instrumented pass took a different step sequence")
+    # the whole pulse, 0 -> tau, graph-replayed: the controller's accept
+    # ratio through the strong-field middle of the pulse, where the timed
+    # region above (the first attempts from t = 0) never rejects
+    prop.set_state(x0)
+    t0 = time.perf_counter()
+    sw, _ = prop.prop_adaptive(0.0, tau, 0.05, TOL, TOL)
+    wall_w = time.perf_counter() - t0
+    whole = {"t0": 0.0, "t1": round(tau, 6), "accepted": sw[0].accepted,
+             "rejected": sw[0].rejected, "attempts": sw[0].attempts,
+             "wall_s": round(wall_w, 4),
+             "steps_per_s": round(sw[0].accepted / wall_w, 3),
+             "attempts_per_s": round(sw[0].attempts / wall_w, 3),
+             "ms_per_attempt": round(wall_w / sw[0].attempts * 1e3, 4)}
     s = st[0]
     return {"wall": wall, "accepted": s.accepted, "rejected": s.rejected,
             "attempts": s.attempts, "t_reached": s.t, "kernel_ms": kms,
             "kernel_launches": klaunch, "kernel_bytes": kbytes, "prop": prop,
-            "tau": tau}
+            "tau": tau, "whole_pulse": whole}
 
 
 S_HE_TEST = [3030, 2799, 2808, 2808]  # inp/He_test.yaml states_in_l
@@ -593,6 +606,7 @@ def main():
                      "bytes_per_launch": r["kernel_bytes"],
                      "launches": r["kernel_launches"]},
     }
+    out["whole_pulse"] = r["whole_pulse"]
     if ref is not None:
         out["reference_mode"] = {"steps_per_s": round(ref["steps_per_s"], 3),
                                  "dt": ref["dt"], "steps": ref["steps"]}

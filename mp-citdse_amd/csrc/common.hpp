@@ -45,8 +45,9 @@ struct TileJob {
 // X_{L+1}) or transpose (dir 1: rows of block L+1, from X_L).  X is the
 // state viewed as a row-major matrix with 2P real columns.
 constexpr int kGmTK = 32;  // K chunk staged in LDS
-// Lanes per row of k_stage for one pulse (they split a row's partials).
-constexpr int kStageSub1 = 4;
+// Lanes per row of the single-pulse stage kernel (k_stage1): they split a
+// row's partial sums and its stage-vector loads, all issued in one round.
+constexpr int kStageSub1 = 16;
 
 struct GemmJob {
   int L;
@@ -112,6 +113,9 @@ struct Ctl {
   int status;      // 0 ok, -1 attempt limit hit
   int nlog;        // log entries written in the current chunk
   int hstride;     // row stride of the per-pulse step / field tables
+  double Fs[kStages];  // the field at the current attempt's 13 stage times
+                       // (frozen-field modes: the same value 13 times),
+                       // set on the device when the attempt is set up
 };
 
 // Optional stores written while stepping (adjoint support).

@@ -1,9 +1,9 @@
 // Host side of the MI355X tdse engine and its C ABI (include/mpci_tdse.h).
 //
-// One RKF78 attempt = 13 x (coupling, k_stage) + k_finalize + k_apply, the
-// coupling being k_gemv_tile for one pulse and an MFMA GEMM for a batch;
+// One RKF78 attempt = 13 x (coupling, stage kernel) + k_finalize + k_apply,
+// the coupling being k_gemv_tile for one pulse and an MFMA GEMM for a batch;
 // one pulse on a small system takes 13 fused k_small_stage launches
-// instead.  The attempt is captured once into a hipGraph and replayed.  Step control lives in
+// instead.  k_finalize also sets the next attempt's 13 stage fields.  The attempt is captured once into a hipGraph and replayed.  Step control lives in
 // device memory (Ctl), so the host only re-launches the graph in chunks and
 // drains the per-step log between chunks.
 #include <hip/hip_ext.h>
@@ -122,7 +122,7 @@ struct mpci_prop {
   SysDesc desc{};
   long long NP = 0;
   int nblk = 0;           // workgroups of the per-row kernels (256 rows / P)
-  int nblk_stage = 0;     // k_stage workgroups (64 rows for one pulse)
+  int nblk_stage = 0;     // stage-kernel workgroups (16 rows for one pulse)
   // one pulse on a small system: one fused launch per stage
   // (k_small_stage, four rows per workgroup) instead of GEMV tile + stage
   bool small = false;
@@ -410,24 +410,39 @@ extern "C" int mpci_system_info(const mpci_system *sys, int *N,
 // Partial-sum layout: the GEMV form writes ncb(L) row partials and nrb(L)
 // column partials per pair; the GEMM form writes `ks` K-split partials per
 // pair and direction.
-static int init_desc(mpci_prop *pr, int ks) {
-  mpci_system *sys = pr->sys;
-  SysDesc &d = pr->desc;
+// Workgroups of the single-pulse stage kernel k_stage1: ceil(s_L / 16)
+// per block, each workgroup inside one block.
+static int stage1_blocks(const mpci_system *sys) {
+  const int rpb = 256 / kStageSub1;
+  int n = 0;
+  for (int L = 0; L < sys->nb; ++L) n += (sys->s[L] + rpb - 1) / rpb;
+  return n;
+}
+
+// Fills d for P pulses (K split ks in the batched form) and returns the
+// number of double2 partials the layout needs.
+static long long fill_desc(const mpci_system *sys, int P, int ks,
+                           SysDesc &d) {
   std::memset(&d, 0, sizeof(d));
   d.nb = sys->nb;
   d.N = sys->N;
-  d.P = pr->P;
+  d.P = P;
   d.gauge = sys->gauge;
   for (int L = 0; L <= sys->nb; ++L) d.off[L] = sys->off[L];
   long long o = 0;
   for (int L = 0; L + 1 < sys->nb; ++L) {
-    d.nrow_part[L] = pr->P == 1 ? sys->ncb[L] : ks;
-    d.ncol_part[L] = pr->P == 1 ? sys->nrb[L] : ks;
+    d.nrow_part[L] = P == 1 ? sys->ncb[L] : ks;
+    d.ncol_part[L] = P == 1 ? sys->nrb[L] : ks;
     d.row_part_off[L] = o;
-    o += (long long)d.nrow_part[L] * sys->s[L] * pr->P;
+    o += (long long)d.nrow_part[L] * sys->s[L] * P;
     d.col_part_off[L] = o;
-    o += (long long)d.ncol_part[L] * sys->s[L + 1] * pr->P;
+    o += (long long)d.ncol_part[L] * sys->s[L + 1] * P;
   }
+  return o;
+}
+
+static int init_desc(mpci_prop *pr, int ks) {
+  const long long o = fill_desc(pr->sys, pr->P, ks, pr->desc);
   TRY(pr->part.alloc(std::max<long long>(o, 1)));
   return MPCI_OK;
 }
@@ -799,8 +814,7 @@ extern "C" int mpci_prop_create(mpci_system *sys, int P, mpci_prop **out) {
   pr->NP = (long long)sys->N * P;
   const int rpb = 256 / P;
   pr->nblk = (sys->N + rpb - 1) / rpb;
-  const int rps = P == 1 ? 256 / kStageSub1 : rpb;  // rows per k_stage block
-  pr->nblk_stage = (sys->N + rps - 1) / rps;
+  pr->nblk_stage = P == 1 ? stage1_blocks(sys) : pr->nblk;
   TRY(init_desc(pr.get(), 1));
   TRY(pr->X.alloc(pr->NP));
   TRY(pr->Xs.alloc(pr->NP));
@@ -947,11 +961,11 @@ static int launch_stage(mpci_prop *pr, const StageArgs &a) {
   const bool last = a.s == kStages - 1;
   if (pr->P == 1) {
     if (last)
-      hipLaunchKernelGGL((k_stage<kStageSub1, true>), grid, block, 0,
-                         pr->stream, a, pr->desc);
+      hipLaunchKernelGGL((k_stage1<true>), grid, block, 0, pr->stream, a,
+                         pr->desc);
     else
-      hipLaunchKernelGGL((k_stage<kStageSub1, false>), grid, block, 0,
-                         pr->stream, a, pr->desc);
+      hipLaunchKernelGGL((k_stage1<false>), grid, block, 0, pr->stream, a,
+                         pr->desc);
   } else if (last) {
     hipLaunchKernelGGL((k_stage<1, true>), grid, block, 0, pr->stream, a,
                        pr->desc);
@@ -981,9 +995,6 @@ static StageArgs stage_args(mpci_prop *pr, int s) {
   a.E = pr->sys->E.p;
   a.red = pr->red.p;
   a.ctl = pr->ctl.p;
-  a.pulses = pr->pulses.p;
-  a.env = pr->env.p;
-  a.Ftab = pr->Ftab.p;
   a.xs_store = pr->store.xs;
   a.NP = pr->NP;
   a.s = s;
@@ -1027,7 +1038,7 @@ static int enqueue_attempt(mpci_prop *pr) {
   hipLaunchKernelGGL(k_finalize, dim3(pr->P), dim3(256), 0, pr->stream,
                      pr->ctl.p, pr->red.p,
                      pr->small ? pr->nblk_small : pr->nblk_stage, pr->P,
-                     pr->hs.p);
+                     pr->hs.p, pr->Ftab.p, pr->pulses.p, pr->env.p);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_apply, dim3(pr->nblk), dim3(256), 0, pr->stream,
                      pr->X.p, pr->Y.p, pr->ctl.p, pr->log.p, pr->logcap,
@@ -1134,6 +1145,17 @@ static int write_ctl(mpci_prop *pr, const std::vector<Ctl> &c) {
   return MPCI_OK;
 }
 
+// Writes the Ctl records a run starts from, then sets their first
+// attempt's stage fields on the device (k_finalize sets the later ones).
+static int start_ctl(mpci_prop *pr, const std::vector<Ctl> &c) {
+  TRY(write_ctl(pr, c));
+  hipLaunchKernelGGL(k_ctl_fields, dim3(pr->P), dim3(64), 0, pr->stream,
+                     pr->ctl.p, pr->P, pr->Ftab.p, pr->pulses.p, pr->env.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(pr->stream));
+  return MPCI_OK;
+}
+
 // Drain the per-chunk log into the caller's arrays and reset the counters.
 static int drain_log(mpci_prop *pr, std::vector<Ctl> &c,
                      std::vector<std::vector<LogEnt>> &acc) {
@@ -1164,7 +1186,7 @@ extern "C" int mpci_prop_rhs(mpci_prop *pr, const double *F, const double *x,
     c[p].fmode = kFieldConst;
     c[p].Fc = F[p];
   }
-  TRY(write_ctl(pr, c));
+  TRY(start_ctl(pr, c));
   StageArgs a = stage_args(pr, 0);
   a.red = nullptr;
   if (pr->small) {
@@ -1209,7 +1231,7 @@ extern "C" int mpci_prop_fixed(mpci_prop *pr, double t0, double dt, int steps,
     x.hstride = steps;
     x.max_attempts = steps;
   }
-  TRY(write_ctl(pr, c));
+  TRY(start_ctl(pr, c));
   std::vector<std::vector<LogEnt>> acc(pr->P);
   int left = steps;
   while (left > 0) {
@@ -1254,7 +1276,7 @@ static int plan_chunk(const mpci_prop *pr, const std::vector<Ctl> &c) {
 
 static int run_controlled(mpci_prop *pr, std::vector<Ctl> &c,
                           std::vector<std::vector<LogEnt>> &acc) {
-  TRY(write_ctl(pr, c));
+  TRY(start_ctl(pr, c));
   for (int n; (n = plan_chunk(pr, c)) > 0;) {
     TRY(run_attempts(pr, n));
     TRY(read_ctl(pr, c));
@@ -1370,7 +1392,7 @@ extern "C" int mpci_multi_adaptive(int n, mpci_prop *const *props,
     m[i].pr = props[i];
     TRY(setup_adaptive(props[i], t0[i], t1[i], h0[i], atol, rtol,
                        max_attempts, 0, m[i].c, m[i].acc));
-    TRY(write_ctl(props[i], m[i].c));
+    TRY(start_ctl(props[i], m[i].c));
   }
   for (;;) {
     int live = 0;
@@ -1563,7 +1585,7 @@ extern "C" int mpci_prop_adjoint(mpci_prop *pr, const double *t0,
   std::vector<std::vector<LogEnt>> acc;
   TRY(setup_adaptive(pr, t0, t1, h0, atol, rtol, max_steps * 8 + 64, 0, c,
                      acc));
-  TRY(write_ctl(pr, c));
+  TRY(start_ctl(pr, c));
   for (int n; (n = plan_chunk(pr, c)) > 0;) {
     int done_max = 0;
     for (const Ctl &x : c) done_max = std::max(done_max, x.accepted);

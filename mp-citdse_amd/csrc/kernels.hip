@@ -10,7 +10,10 @@
 //                    by the stage field, stores k_s and forms the next stage
 //                    input (boost::odeint generic_rk stage algebra); on the
 //                    last stage it forms the order-8 solution, the embedded
-//                    error ratio and |y|^2 partials.
+//                    error ratio and |y|^2 partials (batched form).
+//   k_stage1         the same for one pulse, 16 lanes per row, every load
+//                    in one round; its last stage also runs the step
+//                    control (last-arriving workgroup).
 //   k_finalize       per-pulse reduction + on-device accept/reject control.
 //   k_apply          accepted: x = y / ||y|| (tdse.cpp:194-198) + log entry.
 #include "kernels.hpp"
@@ -988,11 +991,30 @@ __device__ __forceinline__ void sum_both(const double2 *cb, int nc,
   }
 }
 
+// k_s of one (row, pulse) from the coupling sums cs = D^T x_{L-1} and
+// rs = D x_{L+1}, the stage input xs, the energy e and the stage field F
+// (MatVecV / MatVecL, src/tdse/tdse.cpp:14-143).
+__device__ __forceinline__ double2 stage_deriv(int gauge, double e, double2 xs,
+                                               double F, double2 cs,
+                                               double2 rs) {
+  double2 k;
+  if (gauge == 0) {  // velocity: -iE x + F (D^T x_{L-1} - D x_{L+1})
+    k.x = e * xs.y + F * (cs.x - rs.x);
+    k.y = -e * xs.x + F * (cs.y - rs.y);
+  } else {           // length: -i (E x + F (D^T x_{L-1} + D x_{L+1}))
+    const double cx = cs.x + rs.x, cy = cs.y + rs.y;
+    k.x = e * xs.y + F * cy;
+    k.y = -e * xs.x - F * cx;
+  }
+  return k;
+}
+
 // The per-row end of a stage, shared by k_stage and k_small_stage: from
 // the coupling sums cs = D^T x_{L-1} and rs = D x_{L+1} of (row, pulse p)
 // it forms k_s, stores it, and forms the next stage input, or on the
 // thirteenth stage the solution, the embedded error ratio (errv) and
-// |y|^2 (nrm2).
+// |y|^2 (nrm2).  The tableau sums run in stage order, one fma per term,
+// from x (k_stage1 sums in the same order).
 template <bool LAST>
 __device__ __forceinline__ void stage_row(const StageArgs &a,
                                           const SysDesc &d, const Ctl &c,
@@ -1002,29 +1024,14 @@ __device__ __forceinline__ void stage_row(const StageArgs &a,
   const int s = a.s;
   const double2 xs = (s == 0) ? a.X[idx] : a.Xs[idx];
   const double e = a.E[row];
-  double F;
-  if (c.fmode == kFieldTable)
-    F = a.Ftab[(long long)p * c.hstride + c.step];
-  else if (c.fmode == kFieldPulse)
-    F = field_at(a.pulses[p], a.env, c.t + cC[s] * c.hc);
-  else
-    F = c.Fc;
-  double2 k;
-  if (d.gauge == 0) {  // velocity: -iE x + F (D^T x_{L-1} - D x_{L+1})
-    k.x = e * xs.y + F * (cs.x - rs.x);
-    k.y = -e * xs.x + F * (cs.y - rs.y);
-  } else {             // length: -i (E x + F (D^T x_{L-1} + D x_{L+1}))
-    const double cx = cs.x + rs.x, cy = cs.y + rs.y;
-    k.x = e * xs.y + F * cy;
-    k.y = -e * xs.x - F * cx;
-  }
+  const double2 k = stage_deriv(d.gauge, e, xs, c.Fs[s], cs, rs);
   const long long NP = a.NP;
   a.K[s * NP + idx] = k;
   if (c.mode != kModeRhs) {
     const double h = c.hc;
     const double2 x = a.X[idx];
     if (!LAST) {
-      double vx = 1.0 * x.x, vy = 1.0 * x.y;
+      double vx = x.x, vy = x.y;
       // the tableau's zero entries (22 of the 66 below the diagonal
       // before the last row) skip their loads: a uniform branch on a
       // constant-memory value
@@ -1037,19 +1044,19 @@ __device__ __forceinline__ void stage_row(const StageArgs &a,
       for (int j = 0; j < kStages - 1; ++j) {
         if (j < s) {
           const double co = h * cA[s + 1][j];
-          vx += co * kj[j].x;
-          vy += co * kj[j].y;
+          vx = fma(co, kj[j].x, vx);
+          vy = fma(co, kj[j].y, vy);
         }
       }
       const double co = h * cA[s + 1][s];
-      vx += co * k.x;
-      vy += co * k.y;
+      vx = fma(co, k.x, vx);
+      vy = fma(co, k.y, vy);
       a.Xs_next[idx] = make_double2(vx, vy);
       if (a.xs_store)
         a.xs_store[((long long)c.step * (kStages - 1) + s) * NP + idx] =
             make_double2(vx, vy);
     } else {
-      double yx = 1.0 * x.x, yy = 1.0 * x.y, ex = 0.0, ey = 0.0;
+      double yx = x.x, yy = x.y, ex = 0.0, ey = 0.0;
       // k_2 .. k_5 carry zero weight in both b and b - bhat
       double2 kj[kStages];
 #pragma unroll
@@ -1062,10 +1069,10 @@ __device__ __forceinline__ void stage_row(const StageArgs &a,
 #pragma unroll
       for (int j = 0; j < kStages; ++j) {
         const double cb = h * cB[j], cd = h * cDB[j];
-        yx += cb * kj[j].x;
-        yy += cb * kj[j].y;
-        ex += cd * kj[j].x;
-        ey += cd * kj[j].y;
+        yx = fma(cb, kj[j].x, yx);
+        yy = fma(cb, kj[j].y, yy);
+        ex = fma(cd, kj[j].x, ex);
+        ey = fma(cd, kj[j].y, ey);
       }
       a.Y[idx] = make_double2(yx, yy);
       const double rx =
@@ -1078,14 +1085,14 @@ __device__ __forceinline__ void stage_row(const StageArgs &a,
   }
 }
 
-// Stage kernel.  SUB lanes share one (row, pulse): they split the partial
-// sums (every SUB-th partial each) and combine them with a DPP reduction
-// inside the group (quad_perm steps, then row_half_mirror and row_mirror
-// for 8 and 16 lanes), lane 0 of the group doing the rest.  SUB > 1 for
-// one pulse (dozens of partials per row; 4 measured fastest on the He
-// workload, against 8 and 16), 1 for the batched form (K-split partials).
-// LAST: the thirteenth stage (solution, error and norm partials), a
-// separate instantiation so the other twelve carry its registers.
+// Stage kernel of the batched form (SUB = 1: one lane per (row, pulse),
+// which sums the few K-split partials itself; the single-pulse form is
+// k_stage1).  SUB > 1 lanes would split the partial sums (every SUB-th
+// each) and combine them with a DPP reduction inside the group (quad_perm
+// steps, then row_half_mirror and row_mirror for 8 and 16 lanes), lane 0
+// of the group doing the rest.  LAST: the thirteenth stage (solution,
+// error and norm partials), a separate instantiation so the other twelve
+// carry its registers.
 template <int SUB, bool LAST>
 __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
   static_assert(SUB == 1 || SUB == 4 || SUB == 8 || SUB == 16,
@@ -1246,42 +1253,26 @@ template __global__ void k_small_stage<true>(StageArgs, SysDesc,
 
 template __global__ void k_stage<1, false>(StageArgs, SysDesc);
 template __global__ void k_stage<1, true>(StageArgs, SysDesc);
-template __global__ void k_stage<kStageSub1, false>(StageArgs, SysDesc);
-template __global__ void k_stage<kStageSub1, true>(StageArgs, SysDesc);
 
-// ---------------------------------------------------------------------
-// Per-pulse reduction and step control; one workgroup per pulse.
-// ---------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_finalize(Ctl *__restrict__ ctl,
-                                                  const double *__restrict__ red,
-                                                  int nblk, int P,
-                                                  const double *__restrict__ hs) {
-  const int p = blockIdx.x, t = threadIdx.x;
-  if (ctl[p].done) {
-    if (t == 0) ctl[p].accept = 0;
-    return;
-  }
-  __shared__ double se[256], sn[256];
-  double em = 0.0, ns = 0.0;
-  for (int b = t; b < nblk; b += 256) {
-    const double e = red[((long long)b * P + p) * 2];
-    em = e > em ? e : em;
-    ns += red[((long long)b * P + p) * 2 + 1];
-  }
-  se[t] = em;
-  sn[t] = ns;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) {
-      se[t] = se[t] > se[t + w] ? se[t] : se[t + w];
-      sn[t] += sn[t + w];
-    }
-    __syncthreads();
-  }
-  if (t != 0) return;
-  Ctl c = ctl[p];
-  const double E = se[0];
-  c.nrm = sqrt(sn[0]);
+// The field at stage time s of pulse p's current attempt: the frozen
+// per-step value of the fixed mode (src/tdse/tdse.cpp:187), the pulse at
+// t + c_s h (adaptive and schedule modes) or a constant.
+__device__ __forceinline__ double stage_field(const Ctl &c, int s, int p,
+                                              const double *__restrict__ Ftab,
+                                              const PulseDev *__restrict__ pulses,
+                                              const double *__restrict__ env) {
+  if (c.fmode == kFieldTable) return Ftab[(long long)p * c.hstride + c.step];
+  if (c.fmode == kFieldPulse)
+    return field_at(pulses[p], env, c.t + cC[s] * c.hc);
+  return c.Fc;
+}
+
+// The step control of one attempt from its error ratio E and |y|^2:
+// accept / reject, the next step size (adaptive), or the next step of a
+// fixed or schedule run.
+__device__ __forceinline__ Ctl control_step(Ctl c, double E, double n2, int p,
+                                            const double *__restrict__ hs) {
+  c.nrm = sqrt(n2);
   c.err = E;
   c.attempts++;
   if (c.mode == kModeAdaptive) {
@@ -1331,8 +1322,259 @@ __global__ __launch_bounds__(256) void k_finalize(Ctl *__restrict__ ctl,
     else if (c.mode == kModeSchedule)
       c.hc = hs[(long long)p * c.hstride + c.step];
   }
-  ctl[p] = c;
+  return c;
 }
+
+// Reduction of pulse p's per-workgroup (error max, |y|^2) partials in a
+// fixed order and the step control (accept / reject, next step size).  Run
+// by all 256 threads of one workgroup.
+__device__ __forceinline__ void finalize_pulse(
+    Ctl *__restrict__ ctl, const double *__restrict__ red, int nblk, int P,
+    int p, const double *__restrict__ hs, const double *__restrict__ Ftab,
+    const PulseDev *__restrict__ pulses, const double *__restrict__ env) {
+  const int t = threadIdx.x;
+  if (ctl[p].done) {
+    if (t == 0) ctl[p].accept = 0;
+    return;
+  }
+  __shared__ double se[256], sn[256];
+  // thread t takes partials t, t + 256, t + 512, ... in that order, four
+  // in flight at a time
+  double em = 0.0, ns = 0.0;
+  for (int b0 = t; b0 < nblk; b0 += 4 * 256) {
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = b0 + 256 * u;
+      v[u] = b < nblk ? reinterpret_cast<const double2 *>(red)[(long long)b * P + p]
+                      : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (b0 + 256 * u < nblk) {
+        em = v[u].x > em ? v[u].x : em;
+        ns += v[u].y;
+      }
+    }
+  }
+  se[t] = em;
+  sn[t] = ns;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      se[t] = se[t] > se[t + w] ? se[t] : se[t + w];
+      sn[t] += sn[t + w];
+    }
+    __syncthreads();
+  }
+  // thread 0 runs the controller into cl; threads s < 13 then set the next
+  // attempt's stage fields in parallel; thread 0 stores the record
+  __shared__ Ctl cl;
+  if (t == 0) cl = control_step(ctl[p], se[0], sn[0], p, hs);
+  __syncthreads();
+  if (t < kStages && !cl.done) cl.Fs[t] = stage_field(cl, t, p, Ftab, pulses, env);
+  __syncthreads();
+  if (t == 0) ctl[p] = cl;
+}
+
+
+// ---------------------------------------------------------------------
+// Per-pulse reduction and step control; one workgroup per pulse.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_finalize(
+    Ctl *__restrict__ ctl, const double *__restrict__ red, int nblk, int P,
+    const double *__restrict__ hs, const double *__restrict__ Ftab,
+    const PulseDev *__restrict__ pulses, const double *__restrict__ env) {
+  finalize_pulse(ctl, red, nblk, P, blockIdx.x, hs, Ftab, pulses, env);
+}
+
+// one workgroup per pulse, thread s sets stage s's field
+__global__ __launch_bounds__(64) void k_ctl_fields(
+    Ctl *__restrict__ ctl, int P, const double *__restrict__ Ftab,
+    const PulseDev *__restrict__ pulses, const double *__restrict__ env) {
+  const int p = blockIdx.x, s = threadIdx.x;
+  if (p >= P || s >= kStages || ctl[p].done) return;
+  ctl[p].Fs[s] = stage_field(ctl[p], s, p, Ftab, pulses, env);
+}
+
+// Lane J's v within a row of 16 lanes, as seen by the row's lane 0 (DPP
+// row_shl:J; J = 0 is the lane's own value).
+template <int J>
+__device__ __forceinline__ double row_lane(double v) {
+  if constexpr (J == 0)
+    return v;
+  else
+    return dpp_d<0x100 + J>(v);
+}
+
+// Lane 0 of a row adds the tableau terms h w_J k_J, J = 0 .. 11 in order,
+// each k_J fetched from lane J (where it was loaded, or zero), one fma per
+// term as stage_row does: the next stage input (w = a_{s+1,J}, J < s) or,
+// on the last stage, the solution (w = b_J) and the error estimate
+// (w = b_J - bhat_J).
+template <int J, bool LAST>
+__device__ __forceinline__ void row_tableau_terms(double2 kv, int s, double h,
+                                                  double &vx, double &vy,
+                                                  double &ex, double &ey) {
+  if constexpr (J < kStages - 1) {
+    const double kx = row_lane<J>(kv.x), ky = row_lane<J>(kv.y);
+    if (LAST) {
+      const double cb = h * cB[J], cd = h * cDB[J];
+      vx = fma(cb, kx, vx);
+      vy = fma(cb, ky, vy);
+      ex = fma(cd, kx, ex);
+      ey = fma(cd, ky, ey);
+    } else if (J < s) {
+      const double co = h * cA[s + 1][J];
+      vx = fma(co, kx, vx);
+      vy = fma(co, ky, vy);
+    }
+    row_tableau_terms<J + 1, LAST>(kv, s, h, vx, vy, ex, ey);
+  }
+}
+
+// ---------------------------------------------------------------------
+// Single-pulse stage kernel.  SUB = 16 lanes share one row: lane `sub`
+// takes the partial sums q = sub, sub + 16, ... of both coupling sums and
+// the stage derivative k_sub that the stage combination needs (chosen by
+// the stage's bit mask of nonzero tableau weights), lane 0 also the
+// committed state x, the stage input and the energy; every load is issued
+// before any is used, so a stage costs one round of memory latency.  The
+// partial sums are summed over the 16 lanes by the fixed DPP steps of
+// sum16; lane 0 then fetches each k_j from its lane (DPP row_shl) and runs
+// stage_row's algebra with the tableau sums in stage_row's order, so the
+// stage algebra gives stage_row's bits.  The stage field comes with the
+// attempt (Ctl::Fs), not from a per-row evaluation of the pulse.
+// Workgroups are dealt to the blocks in order, ceil(s_L / 16) each, so a
+// workgroup's block is uniform.  The thirteenth stage leaves one (error
+// max, |y|^2) partial per workgroup for k_finalize.
+// ---------------------------------------------------------------------
+// bit j: stage s + 1's input needs k_j (j < s, a_{s+1,j} != 0); row 12
+// (the last stage): the solution or the error estimate needs k_j, j < 12
+__constant__ unsigned cNeed[kStages] = {
+    0x0000, 0x0001, 0x0001, 0x0005, 0x0009, 0x0019, 0x0031, 0x0079,
+    0x00F9, 0x01F9, 0x03E1, 0x03F9, 0x0FE1};
+
+template <bool LAST>
+__global__ __launch_bounds__(256) void k_stage1(StageArgs a, SysDesc d) {
+  constexpr int SUB = kStageSub1, RPB = 256 / SUB;
+  const int t = threadIdx.x, sub = t & (SUB - 1), lr = t / SUB;
+  const int s = a.s;
+  const Ctl c = a.ctl[0];
+  // this workgroup's block L and its rows
+  int L = 0, wg0 = 0;
+  for (;;) {
+    const int n = (d.off[L + 1] - d.off[L] + RPB - 1) / RPB;
+    if ((int)blockIdx.x < wg0 + n || L == d.nb - 1) break;
+    wg0 += n;
+    ++L;
+  }
+  const int sL = d.off[L + 1] - d.off[L];
+  const int i = ((int)blockIdx.x - wg0) * RPB + lr;  // row within block L
+  const int row = d.off[L] + i;
+  double errv = 0.0, nrm2 = 0.0;
+  // uniform over each group of 16 lanes, so the DPP sums stay inside it
+  if (i < sL && !c.done) {
+    const long long NP = a.NP;
+    const bool rhs = c.mode == kModeRhs;
+    const double h = c.hc;
+    // ---- loads, all in one round
+    const int j = sub < kStages - 1 ? sub : 0;
+    const bool need = !rhs && sub < kStages - 1 &&
+                      ((cNeed[LAST ? kStages - 1 : s] >> sub) & 1u);
+    double2 kv = make_double2(0.0, 0.0);  // this lane's k_j
+    if (need) kv = a.K[j * NP + row];
+    double2 x = make_double2(0.0, 0.0), xin = x;
+    double e = 0.0;
+    if (sub == 0) {
+      // the stage input is loaded at stage 0 too (Xs is always a valid
+      // buffer there, and unused), so no load waits for another
+      x = a.X[row];
+      xin = a.Xs[row];
+      e = a.E[row];
+    }
+    const int nc = L > 0 ? d.ncol_part[L - 1] : 0;
+    const int nr = L < d.nb - 1 ? d.nrow_part[L] : 0;
+    const double2 *cb = a.part + (L > 0 ? d.col_part_off[L - 1] : 0) + i;
+    const double2 *rb = a.part + (L < d.nb - 1 ? d.row_part_off[L] : 0) + i;
+    double2 cs = make_double2(0.0, 0.0), rs = cs;
+    const int n = nc > nr ? nc : nr;
+    for (int q0 = sub; q0 < n; q0 += 4 * SUB) {
+      double2 vc[4], vr[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int q = q0 + u * SUB;
+        vc[u] = q < nc ? cb[(long long)q * sL] : make_double2(0.0, 0.0);
+        vr[u] = q < nr ? rb[(long long)q * sL] : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cs.x += vc[u].x;
+        cs.y += vc[u].y;
+        rs.x += vr[u].x;
+        rs.y += vr[u].y;
+      }
+    }
+    // the stage's field, set with the attempt (a uniform load)
+    const double F = a.ctl[0].Fs[s];
+    // ---- the 16-lane coupling sums; the tableau sums from lane 0's x
+    cs.x = sum16(cs.x);
+    cs.y = sum16(cs.y);
+    rs.x = sum16(rs.x);
+    rs.y = sum16(rs.y);
+    double vx = x.x, vy = x.y, ex = 0.0, ey = 0.0;
+    if (!rhs) row_tableau_terms<0, LAST>(kv, s, h, vx, vy, ex, ey);
+    if (sub == 0) {
+      const double2 xs = (s == 0) ? x : xin;
+      const double2 k = stage_deriv(d.gauge, e, xs, F, cs, rs);
+      a.K[s * NP + row] = k;
+      if (!rhs) {
+        if (!LAST) {
+          const double cs1 = h * cA[s + 1][s];
+          vx = fma(cs1, k.x, vx);
+          vy = fma(cs1, k.y, vy);
+          a.Xs_next[row] = make_double2(vx, vy);
+          if (a.xs_store)
+            a.xs_store[((long long)c.step * (kStages - 1) + s) * NP + row] =
+                make_double2(vx, vy);
+        } else {
+          const double cb12 = h * cB[kStages - 1], cd12 = h * cDB[kStages - 1];
+          vx = fma(cb12, k.x, vx);
+          vy = fma(cb12, k.y, vy);
+          ex = fma(cd12, k.x, ex);
+          ey = fma(cd12, k.y, ey);
+          a.Y[row] = make_double2(vx, vy);
+          // k_1 is lane 0's own k_0
+          const double rx =
+              fabs(ex) / (c.atol + c.rtol * (fabs(x.x) + h * fabs(kv.x)));
+          const double ry =
+              fabs(ey) / (c.atol + c.rtol * (fabs(x.y) + h * fabs(kv.y)));
+          errv = rx > ry ? rx : ry;
+          nrm2 = vx * vx + vy * vy;
+        }
+      }
+    }
+  }
+  if (LAST && a.red != nullptr) {
+    __shared__ double se[RPB], sn[RPB];
+    if (sub == 0) {
+      se[lr] = errv;
+      sn[lr] = nrm2;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double em = 0.0, ns = 0.0;
+      for (int r = 0; r < RPB; ++r) {
+        em = se[r] > em ? se[r] : em;
+        ns += sn[r];
+      }
+      a.red[(long long)blockIdx.x * 2 + 0] = em;
+      a.red[(long long)blockIdx.x * 2 + 1] = ns;
+    }
+  }
+}
+template __global__ void k_stage1<false>(StageArgs, SysDesc);
+template __global__ void k_stage1<true>(StageArgs, SysDesc);
 
 __global__ __launch_bounds__(256) void k_apply(double2 *__restrict__ X,
                                                const double2 *__restrict__ Y,

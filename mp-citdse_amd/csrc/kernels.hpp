@@ -7,7 +7,8 @@ namespace mpci {
 
 struct StageArgs {
   const double2 *X;      // committed state (N x P)
-  double2 *Xs;           // stage input (stages 1..12)
+  double2 *Xs;           // stage input (stages 1..12; a valid buffer of
+                         // N x P at stage 0 too, where it is not used)
   double2 *Xs_next;      // where the next stage's input goes: Xs itself
                          // when the coupling ran in an earlier kernel, the
                          // other of two buffers on the fused path
@@ -16,10 +17,7 @@ struct StageArgs {
   const double2 *part;   // coupling partial sums
   const double *E;       // energies (N)
   double *red;           // per-block (error max, |y|^2) partials
-  Ctl *ctl;              // per-pulse control (P)
-  const PulseDev *pulses;
-  const double *env;     // envelope samples of PWL pulses
-  const double *Ftab;    // frozen per-step fields (P x nsteps)
+  Ctl *ctl;              // per-pulse control (P), with the stage fields
   double2 *xs_store;     // optional: stage inputs of every step
   long long NP;
   int s;                 // stage 0..12
@@ -69,12 +67,19 @@ __global__ void k_gemm_coupling(const GemmJob *jobs, const double *Dr,
 template <int SUB, bool LAST>
 __global__ void k_stage(StageArgs a, SysDesc d);
 template <bool LAST>
+__global__ void k_stage1(StageArgs a, SysDesc d);
+template <bool LAST>
 __global__ void k_small_stage(StageArgs a, SysDesc d, const double *Dr,
                               const long long *dr_off, const int *dr_ld,
                               const double *DrT, const long long *drt_off,
                               const int *drt_ld);
 __global__ void k_finalize(Ctl *ctl, const double *red, int nblk, int P,
-                           const double *hs);
+                           const double *hs, const double *Ftab,
+                           const PulseDev *pulses, const double *env);
+// The stage fields Ctl::Fs of every live pulse from its Ctl as the host set
+// it up (the first attempt of a run; k_finalize sets them for the next).
+__global__ void k_ctl_fields(Ctl *ctl, int P, const double *Ftab,
+                             const PulseDev *pulses, const double *env);
 __global__ void k_apply(double2 *X, const double2 *Y, const Ctl *ctl,
                         LogEnt *log, int logcap, int track, int N, int P,
                         StepStore st);

@@ -352,7 +352,11 @@ def test_pwl_envelope_change_between_graph_runs(mpci, oracle):
     """set_pulse with a longer piecewise-linear envelope moves the device
     envelope store, which a captured step refers to: the graph is recaptured
     and graph runs stay bit-equal to eager runs (then a same-length envelope
-    is rewritten in place), each run matching the oracle."""
+    is rewritten in place), each run matching the oracle's map replayed on
+    the run's own accepted steps to 1e-10.  (Two independent adaptive runs
+    through these jagged envelopes differ by ~1e-6: at an envelope kink the
+    order-7 and order-8 solutions share an O(h^2) error that their
+    difference, the controller's estimate, cannot see.)"""
     from mpci import synth
     g, o = build(mpci, oracle, [40, 52, 33], "v", seed=23)
     w, T = 0.7, 20.0
@@ -360,20 +364,24 @@ def test_pwl_envelope_change_between_graph_runs(mpci, oracle):
             synth.uniform(202, 0, 40, 0.0, 0.05),
             synth.uniform(203, 0, 40, 0.0, 0.05)]
     x0 = synth.ground_state(g.N)
-    runs = {}
+    runs, logs = {}, {}
     for graph in (True, False):
         prop = mpci.Propagator(g)
         prop.set_graph(graph)
-        out = []
+        out, hl = [], []
         for env in envs:
             prop.set_state(x0)
             prop.set_pulse(mpci.Pulse.pwl(env, w, T))
-            prop.prop_adaptive(0.0, T, 0.05, 1e-10, 1e-10)
+            st, log = prop.prop_adaptive(0.0, T, 0.05, 1e-10, 1e-10,
+                                         log_cap=4096)
+            assert st[0].t == T and st[0].accepted < 4096
             out.append(prop.get_state())
-        runs[graph] = out
+            hl.append(log[0, :st[0].accepted, 1].copy())
+        runs[graph], logs[graph] = out, hl
         prop.close()
     for k, env in enumerate(envs):
         assert np.array_equal(runs[True][k], runs[False][k]), k
-        xo, _, _ = o.prop_adaptive(oracle.Pulse.pwl(env, w, T), 0.0, T, 0.05,
-                                   1e-10, 1e-10, x0)
-        assert np.abs(runs[True][k] - xo).max() < 1e-10
+        assert np.array_equal(logs[True][k], logs[False][k]), k
+        xr = o.prop_schedule(oracle.Pulse.pwl(env, w, T), 0.0, logs[True][k],
+                             x0)
+        assert np.abs(runs[True][k] - xr).max() < 1e-10, k

@@ -51,7 +51,24 @@ def test_no_scratch(device_asm):
 
 
 def test_expected_kernels_present(device_asm):
-    for k in ("k_gemv_tile", "k_stage", "k_finalize", "k_apply",
+    for k in ("k_gemv_tile", "k_stage", "k_stage1", "k_finalize", "k_apply",
               "k_adj_stage", "k_adj_init", "k_dot_partial",
               "k_gemm_coupling"):
         assert k in device_asm, k
+
+
+def test_stage_need_masks_match_tableau(oracle):
+    """k_stage1 loads only the k_j whose tableau weight is nonzero, chosen
+    by a per-stage bit mask written into the kernel source: the masks must
+    equal the zero pattern of the oracle's Fehlberg 7(8) tableau (which the
+    oracle tests pin by the exact order conditions)."""
+    src = open(SRC).read()
+    m = re.search(r"cNeed\[kStages\]\s*=\s*\{([^}]*)\}", src)
+    assert m, "cNeed table not found"
+    masks = [int(v, 16) for v in re.findall(r"0x[0-9A-Fa-f]+", m.group(1))]
+    a, b, db, _ = oracle.tableau()
+    want = []
+    for s in range(12):  # stage s forms stage s + 1's input from k_j, j < s
+        want.append(sum(1 << j for j in range(s) if a[s + 1][j] != 0.0))
+    want.append(sum(1 << j for j in range(12) if b[j] != 0.0 or db[j] != 0.0))
+    assert masks == want, [hex(v) for v in want]
