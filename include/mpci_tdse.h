@@ -132,6 +132,22 @@ int mpci_prop_set_pulse(mpci_prop *prop, int p, const mpci_pulse *pulse);
 int mpci_prop_rhs(mpci_prop *prop, const double *F, const double *x,
                   double *dxdt);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops): the
+ * RHS of one pulse, dxdt = f(x) at constant field F, as mpci_prop_rhs with
+ * P = 1 computes it on the large-system path (the same kernels and
+ * summation order), but with x and dxdt the caller's own device buffers of
+ * 2N doubles (16-byte aligned) and no hipMalloc, hipFree or allocating
+ * library call.  The work is enqueued on `stream` (a hipStream_t; NULL is
+ * the null stream) and the call returns without waiting.  The coupling
+ * partials live in a fixed per-device scratch array of 10 MiB inside the
+ * library (the He shape of 4 blocks of 4000 states needs 9.2 MB); a system
+ * that needs more is refused with MPCI_ERR_ARG.  Every call on one device
+ * shares that array: calls whose work overlaps on a device (other streams,
+ * other threads) race and must be ordered by the caller. */
+int mpci_system_rhs_noalloc(const mpci_system *sys, double F, const double *x,
+                            double *dxdt, void *stream);
+
 /* Reference mode (tdse::propV / propL loop, tdse.cpp:186-204): `steps`
  * fixed RKF78 steps of size dt with the field frozen per step at
  * F[p * steps + st] (the caller evaluates field(pars, t + dt)), each step

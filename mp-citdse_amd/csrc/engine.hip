@@ -16,6 +16,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -1207,6 +1208,100 @@ extern "C" int mpci_prop_rhs(mpci_prop *pr, const double *F, const double *x,
     }
   for (auto &x : c) x.done = 1;
   return write_ctl(pr, c);
+}
+
+// ---------------------------------------------------------------------
+// One RHS evaluation that allocates nothing (mpci_system_rhs_noalloc).
+// Its scratch, device memory beyond the caller's input and output, is this
+// file-scope array: a Ctl record at offset 0 (mode RHS, constant field),
+// then the single-pulse GEMV partial sums.  Every device the library runs
+// this on holds one copy until the process ends.  Host code never uses the
+// array's name as an address (on the host it is only a stand-in for the
+// symbol): the device address comes from hipGetSymbolAddress, once per
+// device with that device current, and is kept in g_scratch_ptr.
+// ---------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = 10u << 20;  // 10 MiB
+constexpr size_t kNoAllocPartOffset = 512;          // partials after the Ctl
+static_assert(sizeof(Ctl) <= kNoAllocPartOffset, "Ctl must fit its slot");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on device dev, which must be current.
+int scratch_ptr(int dev, void **out) {
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return fail(MPCI_ERR_ARG, "mpci_system_rhs_noalloc: device ordinal " +
+                                  std::to_string(dev) + " beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(MPCI_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                    hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MPCI_OK;
+}
+}  // namespace
+
+extern "C" int mpci_system_rhs_noalloc(const mpci_system *sys, double F,
+                                       const double *x, double *dxdt,
+                                       void *stream) {
+  if (!sys || !x || !dxdt)
+    return fail(MPCI_ERR_ARG, "mpci_system_rhs_noalloc: NULL argument");
+  if ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(dxdt)) &
+      15)
+    return fail(MPCI_ERR_ARG, "mpci_system_rhs_noalloc: x and dxdt must be "
+                              "16-byte aligned device pointers");
+  SysDesc d;
+  const long long npart = fill_desc(sys, 1, 1, d);
+  const long long need =
+      (long long)kNoAllocPartOffset + npart * (long long)sizeof(double2);
+  if (need > (long long)kNoAllocScratchBytes)
+    return fail(MPCI_ERR_ARG,
+                "mpci_system_rhs_noalloc: this system's coupling partials "
+                "need " + std::to_string(need) + " bytes of scratch; the "
+                "fixed scratch holds " +
+                std::to_string(kNoAllocScratchBytes) +
+                " (use mpci_prop_rhs, which allocates)");
+  HIPCHK(hipSetDevice(sys->device));
+  void *base = nullptr;
+  TRY(scratch_ptr(sys->device, &base));
+  Ctl *ctl = static_cast<Ctl *>(base);
+  double2 *part =
+      reinterpret_cast<double2 *>(static_cast<char *>(base) + kNoAllocPartOffset);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // the Ctl record is rewritten by a kernel (no memset node under capture);
+  // every partial slot is written by exactly one GEMV unit before it is read
+  hipLaunchKernelGGL(k_rhs_ctl, dim3(1), dim3(64), 0, st, ctl, F);
+  HIPCHK(hipGetLastError());
+  const double2 *xv = reinterpret_cast<const double2 *>(x);
+  if (!sys->jobs.empty()) {
+    const int njobs = (int)sys->jobs.size();
+    hipLaunchKernelGGL(k_gemv_tile,
+                       dim3((unsigned)std::min(njobs, sys->gv_grid)),
+                       dim3(kGvThreads), 0, st, sys->djobs.p, njobs, sys->Dg.p,
+                       xv, part, d, ctl);
+    HIPCHK(hipGetLastError());
+  }
+  StageArgs a{};
+  a.X = xv;
+  a.Xs = const_cast<double2 *>(xv);  // read (not used) at stage 0
+  a.K = reinterpret_cast<double2 *>(dxdt);
+  a.part = part;
+  a.E = sys->E.p;
+  a.ctl = ctl;
+  a.NP = sys->N;
+  a.s = 0;
+  hipLaunchKernelGGL((k_stage1<false>), dim3((unsigned)stage1_blocks(sys)),
+                     dim3(256), 0, st, a, d);
+  HIPCHK(hipGetLastError());
+  return MPCI_OK;
 }
 
 extern "C" int mpci_prop_fixed(mpci_prop *pr, double t0, double dt, int steps,

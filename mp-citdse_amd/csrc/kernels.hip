@@ -1775,6 +1775,18 @@ __global__ __launch_bounds__(256) void k_weighted_pop(
   if (t == 0) out[o * P + p] = s[0];
 }
 
+// One RHS evaluation's control record: mode RHS, constant field F, live.
+__global__ void k_rhs_ctl(Ctl *ctl, double F) {
+  if (threadIdx.x == 0) {
+    Ctl c{};
+    c.mode = kModeRhs;
+    c.fmode = kFieldConst;
+    c.Fc = F;
+    for (int s = 0; s < kStages; ++s) c.Fs[s] = F;
+    ctl[0] = c;
+  }
+}
+
 // ---------------------------------------------------------------------
 // Measurement probes: the ceilings this device reaches in practice, which
 // the roofline fractions are also quoted against.  k_probe_read streams a

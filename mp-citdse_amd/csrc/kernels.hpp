@@ -93,6 +93,8 @@ __global__ void k_adj_init(AdjArgs a, SysDesc d);
 __global__ void k_adj_stage(AdjArgs a, SysDesc d);
 __global__ void k_weighted_pop(const double2 *X, const double *W, int N, int P,
                                double *out);
+// Ctl record of one RHS evaluation at constant field F (mode RHS).
+__global__ void k_rhs_ctl(Ctl *ctl, double F);
 __global__ void k_probe_read(const double2 *a, long long n, double *out);
 __global__ void k_probe_mfma(double *out, int iters);
 

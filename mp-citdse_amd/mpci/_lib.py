@@ -82,6 +82,7 @@ _SIGS = [
     ("mpci_prop_get_state", C.c_int, [_vp, C.c_int, _dp]),
     ("mpci_prop_set_pulse", C.c_int, [_vp, C.c_int, C.POINTER(MpciPulse)]),
     ("mpci_prop_rhs", C.c_int, [_vp, _dp, _dp, _dp]),
+    ("mpci_system_rhs_noalloc", C.c_int, [_vp, C.c_double, _vp, _vp, _vp]),
     ("mpci_prop_fixed", C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _dp,
                                   C.c_int, _dp]),
     ("mpci_prop_adaptive", C.c_int, [_vp, _dp, _dp, _dp, C.c_double,
