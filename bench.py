@@ -251,18 +251,38 @@ def run_multi(mpci, synth, sysg, P=32, attempts=120):
         prop.prop_adaptive(0.0, t1, 0.05, TOL, TOL, max_attempts=attempts,
                            allow_limit=True)
         seq.append(time.perf_counter() - t0)
+    # every member's attempt in the same launches (one interleaved GEMM job
+    # list, one stage / finalize / apply launch per attempt)
+    reset()
+    mpci.multi_adaptive(props, [0.0, 0.0], t1s, [0.05, 0.05], TOL, TOL,
+                        max_attempts=5, allow_limit=True, grouped=True)
+    reset()
+    t0 = time.perf_counter()
+    stg = mpci.multi_adaptive(props, [0.0, 0.0], t1s, [0.05, 0.05], TOL, TOL,
+                              max_attempts=attempts, allow_limit=True,
+                              grouped=True)
+    wall_g = time.perf_counter() - t0
     flops = 0.0
     for (g, _, _), sm in zip(members, st):
         flops += sum(x.attempts for x in sm) * 13 * 8.0 * g.dipole_elements
     acc = [sum(x.accepted for x in sm) for sm in st]
+    if [sum(x.accepted for x in sm) for sm in stg] != acc:
+        raise RuntimeError("grouped run took a different step sequence")
     tf = flops / wall / 1e12
+    tfg = flops / wall_g / 1e12
     return {"P_h": P, "P_he": P, "attempts": attempts,
             "h_blocks": len(S_H), "h_states_per_block": S_H[0],
             "accepted_h": acc[0], "accepted_he": acc[1],
             "pulse_steps_per_s": sum(acc) / wall, "wall_s": wall,
             "sequential_wall_s": sum(seq),
             "overlap_speedup": sum(seq) / wall,
-            "mfma_tflops_wall": tf, "mfma_frac_wall": tf / 78.6}
+            "mfma_tflops_wall": tf, "mfma_frac_wall": tf / 78.6,
+            "mode": "per-stream members (each on its own stream and graph)",
+            "grouped_wall_s": wall_g,
+            "grouped_overlap_speedup": sum(seq) / wall_g,
+            "grouped_mfma_frac_wall": tfg / 78.6,
+            "grouped_mode": "one launch per kernel of the attempt over both "
+                            "members (one GEMM job list, shortest K first)"}
 
 
 S_C1 = [198] * 4

@@ -180,6 +180,21 @@ int mpci_multi_adaptive(int n, mpci_prop *const *props,
                         const double *const *h0, double atol, double rtol,
                         int max_attempts, mpci_stats *const *stats);
 
+/* The same propagation with every member's attempt in the same launches
+ * (extension): one grouped GEMM launch per stage over an interleaved list
+ * of all members' panels, one stage launch over all members' rows, one
+ * step-control and one apply launch over all their pulses, captured once
+ * as a graph.  Each member keeps its own K split, so its results equal a
+ * separate mpci_prop_adaptive call bit for bit.  1 <= n <= 8; every
+ * member must be a batched propagator (P >= 2) with two or more blocks on
+ * the same device, otherwise MPCI_ERR_NOTSUP (use mpci_multi_adaptive). */
+int mpci_multi_adaptive_grouped(int n, mpci_prop *const *props,
+                                const double *const *t0,
+                                const double *const *t1,
+                                const double *const *h0, double atol,
+                                double rtol, int max_attempts,
+                                mpci_stats *const *stats);
+
 /* Replays the adaptive map over n given steps hs[p * n + k] from t0[p]
  * (no controller): used for frozen-schedule finite differences. */
 int mpci_prop_schedule(mpci_prop *prop, const double *t0, int n,

@@ -57,6 +57,7 @@ struct GemmJob {
   int k0, k1;   // K range [k0, k1)
   int q;        // partial slot (K-split index)
   int ld;       // row stride of D_L in the row-major copy (even)
+  int member;   // grouped launches: the member the panel belongs to
 };
 
 // Per-propagator description of the blocks and the partial-sum buffers.

@@ -193,8 +193,9 @@ __device__ __forceinline__ UnitGeo unit_geo(const TileJob *jobs, int u,
 
 __global__ __launch_bounds__(kGvThreads) void k_gemv_tile(
     const TileJob *__restrict__ jobs, int njobs, const double *__restrict__ Dg,
-    const double2 *__restrict__ X, double2 *__restrict__ part, SysDesc d,
+    const double2 *__restrict__ X, double2 *__restrict__ part, const SysDesc *__restrict__ dp,
     const Ctl *__restrict__ ctl) {
+  const SysDesc &d = *dp;
   if (ctl[0].done) return;
   const int G = gridDim.x;
   int u = blockIdx.x;
@@ -364,7 +365,8 @@ template <int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_coupling(
     const GemmJob *__restrict__ jobs, const double *__restrict__ Dr,
     const long long *__restrict__ dr_off, const double *__restrict__ X,
-    double *__restrict__ part, SysDesc d, const Ctl *__restrict__ ctl) {
+    double *__restrict__ part, const SysDesc *__restrict__ dp, const Ctl *__restrict__ ctl) {
+  const SysDesc &d = *dp;
   constexpr int TM = 32 * WM, TN = 32 * WN, TK = kGmTK;
   constexpr int NTH = 64 * WM * WN;
   constexpr int LDA = TM + 2, LDB = TN + 2;  // padded LDS rows
@@ -512,7 +514,8 @@ template <int WM, int WN, int TK, int FM, int FN>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_db(
     const GemmJob *__restrict__ jobs, const double *__restrict__ Dr,
     const long long *__restrict__ dr_off, const double *__restrict__ X,
-    double *__restrict__ part, SysDesc d, const Ctl *__restrict__ ctl) {
+    double *__restrict__ part, const SysDesc *__restrict__ dp, const Ctl *__restrict__ ctl) {
+  const SysDesc &d = *dp;
   constexpr int TM = 16 * FM * WM, TN = 16 * FN * WN;
   constexpr int NTH = 64 * WM * WN;
   constexpr int LDA = TM + 2, LDB = TN + 2;  // padded, even LDS rows
@@ -833,21 +836,19 @@ __device__ __forceinline__ void gemm_lds_body(
       }
 }
 
-// eight-wave tiles: registers held to 80 so that three workgroups (the
-// LDS limit, 48 KiB each) share a compute unit
+// One panel job of the direct-to-LDS GEMM (the body of k_gemm_lds and of
+// its grouped form k_gemm_lds_g).
 template <int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) __attribute__((
-    amdgpu_waves_per_eu(WM * WN == 8 ? 6 : 1))) void k_gemm_lds(
-    const GemmJob *__restrict__ jobs, const double *__restrict__ Dr,
+__device__ __forceinline__ void gemm_lds_panel(
+    const GemmJob &jb, const double *__restrict__ Dr,
     const long long *__restrict__ dr_off, const double *__restrict__ X,
-    double *__restrict__ part, SysDesc d, const Ctl *__restrict__ ctl) {
+    double *__restrict__ part, const SysDesc &d, const Ctl *__restrict__ ctl) {
   constexpr int TM = 32 * WM, TN = 32 * WN, TK = 16;
   constexpr int NTH = 64 * WM * WN;
   __shared__ __align__(16) double As0[TM * TK], As1[TM * TK];
   __shared__ __align__(16) double Bs0[TK * TN], Bs1[TK * TN];
   __shared__ int live;
 
-  const GemmJob jb = jobs[blockIdx.x];
   const int t = threadIdx.x;
   const int P2 = 2 * d.P;
   // skip the panel when every pulse it covers has finished
@@ -883,10 +884,37 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((
     gemm_lds_body<WM, WN, 1>(jb, rsA, rsB, out, Mdim, P2, As0, As1, Bs0, Bs1);
 }
 
+// eight-wave tiles: registers held to 80 so that three workgroups (the
+// LDS limit, 48 KiB each) share a compute unit
+template <int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) __attribute__((
+    amdgpu_waves_per_eu(WM * WN == 8 ? 6 : 1))) void k_gemm_lds(
+    const GemmJob *__restrict__ jobs, const double *__restrict__ Dr,
+    const long long *__restrict__ dr_off, const double *__restrict__ X,
+    double *__restrict__ part, const SysDesc *__restrict__ dp,
+    const Ctl *__restrict__ ctl) {
+  const GemmJob jb = jobs[blockIdx.x];
+  gemm_lds_panel<WM, WN>(jb, Dr, dr_off, X, part, *dp, ctl);
+}
+
+// Grouped form: one launch over the panels of several members (each job
+// names its member), every member's stage input X (stage 0) or Xs.
+template <int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) __attribute__((
+    amdgpu_waves_per_eu(WM * WN == 8 ? 6 : 1))) void k_gemm_lds_g(
+    const GemmJob *__restrict__ jobs, GroupGemm g, int s) {
+  const GemmJob jb = jobs[blockIdx.x];
+  const int i = jb.member;
+  gemm_lds_panel<WM, WN>(jb, g.Dr[i], g.dr_off[i], s == 0 ? g.X[i] : g.Xs[i],
+                         g.part[i], *g.d[i], g.ctl[i]);
+}
+
 #define MPCI_GEMM_LDS(WM, WN)                                                 \
   template __global__ void k_gemm_lds<WM, WN>(                               \
       const GemmJob *, const double *, const long long *, const double *,    \
-      double *, SysDesc, const Ctl *);
+      double *, const SysDesc *, const Ctl *);                               \
+  template __global__ void k_gemm_lds_g<WM, WN>(const GemmJob *, GroupGemm, \
+                                                 int);
 MPCI_GEMM_LDS(2, 1)
 MPCI_GEMM_LDS(4, 1)
 MPCI_GEMM_LDS(2, 2)
@@ -898,7 +926,7 @@ MPCI_GEMM_LDS(4, 4)
 #define MPCI_GEMM_DB(WM, WN, TK, FM, FN)                                      \
   template __global__ void k_gemm_db<WM, WN, TK, FM, FN>(                    \
       const GemmJob *, const double *, const long long *, const double *,    \
-      double *, SysDesc, const Ctl *);
+      double *, const SysDesc *, const Ctl *);
 MPCI_GEMM_DB(2, 1, 16, 2, 2)
 MPCI_GEMM_DB(4, 1, 16, 2, 2)
 MPCI_GEMM_DB(2, 2, 16, 2, 2)
@@ -920,19 +948,19 @@ MPCI_GEMM_DB(2, 2, 32, 4, 4)
 template __global__ void k_gemm_coupling<2, 2>(const GemmJob *, const double *,
                                                const long long *,
                                                const double *, double *,
-                                               SysDesc, const Ctl *);
+                                               const SysDesc *, const Ctl *);
 template __global__ void k_gemm_coupling<2, 4>(const GemmJob *, const double *,
                                                const long long *,
                                                const double *, double *,
-                                               SysDesc, const Ctl *);
+                                               const SysDesc *, const Ctl *);
 template __global__ void k_gemm_coupling<4, 2>(const GemmJob *, const double *,
                                                const long long *,
                                                const double *, double *,
-                                               SysDesc, const Ctl *);
+                                               const SysDesc *, const Ctl *);
 template __global__ void k_gemm_coupling<4, 4>(const GemmJob *, const double *,
                                                const long long *,
                                                const double *, double *,
-                                               SysDesc, const Ctl *);
+                                               const SysDesc *, const Ctl *);
 
 // ---------------------------------------------------------------------
 // Stage kernel.  Thread (row, p) with rows_per_block = 256 / P.
@@ -1093,8 +1121,10 @@ __device__ __forceinline__ void stage_row(const StageArgs &a,
 // of the group doing the rest.  LAST: the thirteenth stage (solution,
 // error and norm partials), a separate instantiation so the other twelve
 // carry its registers.
+// The rows of workgroup blk (the body of k_stage and of its grouped form).
 template <int SUB, bool LAST>
-__global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
+__device__ __forceinline__ void stage_block(const StageArgs &a,
+                                            const SysDesc &d, int blk) {
   static_assert(SUB == 1 || SUB == 4 || SUB == 8 || SUB == 16,
                 "lanes per row: 1, 4, 8 or 16");
   const int P = d.P;
@@ -1103,7 +1133,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
   const int tt = t / SUB;
   const int rpb = (256 / SUB) / P;
   const int lr = tt / P, p = tt - lr * P;
-  const int row = blockIdx.x * rpb + lr;
+  const int row = blk * rpb + lr;
   const bool active_lane = (lr < rpb) && (row < d.N);
   double errv = 0.0, nrm2 = 0.0;
   // every lane of a SUB group has the same row and pulse, so the group is
@@ -1160,11 +1190,32 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
         em = v > em ? v : em;
         ns += sn[(r * P + t) * SUB];
       }
-      a.red[((long long)blockIdx.x * P + t) * 2 + 0] = em;
-      a.red[((long long)blockIdx.x * P + t) * 2 + 1] = ns;
+      a.red[((long long)blk * P + t) * 2 + 0] = em;
+      a.red[((long long)blk * P + t) * 2 + 1] = ns;
     }
   }
 }
+
+template <int SUB, bool LAST>
+__global__ __launch_bounds__(256) void k_stage(StageArgs a,
+                                               const SysDesc *__restrict__ dp) {
+  stage_block<SUB, LAST>(a, *dp, blockIdx.x);
+}
+
+// Grouped form: workgroups wg_stage0 .. wg_stage0 + nblk_stage - 1 of the
+// launch run stage s of member i.
+template <bool LAST>
+__global__ __launch_bounds__(256) void k_stage_g(
+    const GroupMember *__restrict__ m, int nm, int s) {
+  int i = 0;
+  while (i + 1 < nm && (int)blockIdx.x >= m[i + 1].wg_stage0) ++i;
+  StageArgs a = m[i].sa;
+  a.s = s;
+  if (s > 0) a.Xs_next = a.Xs;
+  stage_block<1, LAST>(a, *m[i].d, (int)blockIdx.x - m[i].wg_stage0);
+}
+template __global__ void k_stage_g<false>(const GroupMember *, int, int);
+template __global__ void k_stage_g<true>(const GroupMember *, int, int);
 // ---------------------------------------------------------------------
 // One pulse on a small system (dipoles that stay in cache): each stage in
 // one launch instead of the GEMV tile + k_stage pair.  One wave per row
@@ -1180,10 +1231,11 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a, SysDesc d) {
 // ---------------------------------------------------------------------
 template <bool LAST>
 __global__ __launch_bounds__(256) void k_small_stage(
-    StageArgs a, SysDesc d, const double *__restrict__ Dr,
+    StageArgs a, const SysDesc *__restrict__ dp, const double *__restrict__ Dr,
     const long long *__restrict__ dr_off, const int *__restrict__ dr_ld,
     const double *__restrict__ DrT, const long long *__restrict__ drt_off,
     const int *__restrict__ drt_ld) {
+  const SysDesc &d = *dp;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int row = blockIdx.x * 4 + w;
   const Ctl &c = a.ctl[0];
@@ -1240,19 +1292,19 @@ __global__ __launch_bounds__(256) void k_small_stage(
     }
   }
 }
-template __global__ void k_small_stage<false>(StageArgs, SysDesc,
+template __global__ void k_small_stage<false>(StageArgs, const SysDesc *,
                                               const double *,
                                               const long long *, const int *,
                                               const double *,
                                               const long long *, const int *);
-template __global__ void k_small_stage<true>(StageArgs, SysDesc,
+template __global__ void k_small_stage<true>(StageArgs, const SysDesc *,
                                              const double *,
                                              const long long *, const int *,
                                              const double *,
                                              const long long *, const int *);
 
-template __global__ void k_stage<1, false>(StageArgs, SysDesc);
-template __global__ void k_stage<1, true>(StageArgs, SysDesc);
+template __global__ void k_stage<1, false>(StageArgs, const SysDesc *);
+template __global__ void k_stage<1, true>(StageArgs, const SysDesc *);
 
 // The field at stage time s of pulse p's current attempt: the frozen
 // per-step value of the fixed mode (src/tdse/tdse.cpp:187), the pulse at
@@ -1388,6 +1440,16 @@ __global__ __launch_bounds__(256) void k_finalize(
   finalize_pulse(ctl, red, nblk, P, blockIdx.x, hs, Ftab, pulses, env);
 }
 
+// Grouped form: workgroup pulse0 + p runs pulse p of member i.
+__global__ __launch_bounds__(256) void k_finalize_g(
+    const GroupMember *__restrict__ m, int nm) {
+  int i = 0;
+  while (i + 1 < nm && (int)blockIdx.x >= m[i + 1].pulse0) ++i;
+  const GroupMember &g = m[i];
+  finalize_pulse(g.sa.ctl, g.sa.red, g.nblk_stage, g.P,
+                 (int)blockIdx.x - g.pulse0, g.hs, g.Ftab, g.pulses, g.env);
+}
+
 // one workgroup per pulse, thread s sets stage s's field
 __global__ __launch_bounds__(64) void k_ctl_fields(
     Ctl *__restrict__ ctl, int P, const double *__restrict__ Ftab,
@@ -1456,7 +1518,8 @@ __constant__ unsigned cNeed[kStages] = {
     0x00F9, 0x01F9, 0x03E1, 0x03F9, 0x0FE1};
 
 template <bool LAST>
-__global__ __launch_bounds__(256) void k_stage1(StageArgs a, SysDesc d) {
+__global__ __launch_bounds__(256) void k_stage1(StageArgs a, const SysDesc *__restrict__ dp) {
+  const SysDesc &d = *dp;
   constexpr int SUB = kStageSub1, RPB = 256 / SUB;
   const int t = threadIdx.x, sub = t & (SUB - 1), lr = t / SUB;
   const int s = a.s;
@@ -1573,19 +1636,21 @@ __global__ __launch_bounds__(256) void k_stage1(StageArgs a, SysDesc d) {
     }
   }
 }
-template __global__ void k_stage1<false>(StageArgs, SysDesc);
-template __global__ void k_stage1<true>(StageArgs, SysDesc);
+template __global__ void k_stage1<false>(StageArgs, const SysDesc *);
+template __global__ void k_stage1<true>(StageArgs, const SysDesc *);
 
-__global__ __launch_bounds__(256) void k_apply(double2 *__restrict__ X,
-                                               const double2 *__restrict__ Y,
-                                               const Ctl *__restrict__ ctl,
-                                               LogEnt *__restrict__ log,
-                                               int logcap, int track, int N,
-                                               int P, StepStore st) {
+// The rows of workgroup blk of k_apply (also its grouped form).
+__device__ __forceinline__ void apply_block(double2 *__restrict__ X,
+                                            const double2 *__restrict__ Y,
+                                            const Ctl *__restrict__ ctl,
+                                            LogEnt *__restrict__ log,
+                                            int logcap, int track, int N,
+                                            int P, const StepStore &st,
+                                            int blk) {
   const int rpb = 256 / P;
   const int t = threadIdx.x;
   const int lr = t / P, p = t - lr * P;
-  const int row = blockIdx.x * rpb + lr;
+  const int row = blk * rpb + lr;
   if (lr >= rpb || row >= N) return;
   const Ctl &c = ctl[p];
   if (!c.accept) return;
@@ -1607,6 +1672,25 @@ __global__ __launch_bounds__(256) void k_apply(double2 *__restrict__ X,
     const int slot = c.accepted / st.ckpt_every;
     if (slot < st.ckpt_cap) st.ckpt[(long long)slot * NP + idx] = xn;
   }
+}
+
+__global__ __launch_bounds__(256) void k_apply(double2 *__restrict__ X,
+                                               const double2 *__restrict__ Y,
+                                               const Ctl *__restrict__ ctl,
+                                               LogEnt *__restrict__ log,
+                                               int logcap, int track, int N,
+                                               int P, StepStore st) {
+  apply_block(X, Y, ctl, log, logcap, track, N, P, st, blockIdx.x);
+}
+
+// Grouped form: workgroups wg_apply0 .. of the launch apply member i.
+__global__ __launch_bounds__(256) void k_apply_g(
+    const GroupMember *__restrict__ m, int nm) {
+  int i = 0;
+  while (i + 1 < nm && (int)blockIdx.x >= m[i + 1].wg_apply0) ++i;
+  const GroupMember &g = m[i];
+  apply_block(g.X, g.sa.Y, g.sa.ctl, g.log, g.logcap, g.track, g.N, g.P,
+              g.st, (int)blockIdx.x - g.wg_apply0);
 }
 
 // ---------------------------------------------------------------------
@@ -1669,7 +1753,8 @@ __global__ __launch_bounds__(256) void k_reduce_final(
   if (t == 0) out[p] = sv[0];
 }
 
-__global__ __launch_bounds__(256) void k_adj_init(AdjArgs a, SysDesc d) {
+__global__ __launch_bounds__(256) void k_adj_init(AdjArgs a, const SysDesc *__restrict__ dp) {
+  const SysDesc &d = *dp;
   const int P = d.P;
   const int rpb = 256 / P;
   const int t = threadIdx.x;
@@ -1695,7 +1780,8 @@ __global__ __launch_bounds__(256) void k_adj_init(AdjArgs a, SysDesc d) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_adj_stage(AdjArgs a, SysDesc d) {
+__global__ __launch_bounds__(256) void k_adj_stage(AdjArgs a, const SysDesc *__restrict__ dp) {
+  const SysDesc &d = *dp;
   const int P = d.P;
   const int rpb = 256 / P;
   const int t = threadIdx.x;

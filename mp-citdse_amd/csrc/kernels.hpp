@@ -46,30 +46,74 @@ struct AdjArgs {
   int pad;
 };
 
+// The coupling operands of every member of a grouped GEMM launch, passed
+// in the kernel arguments (so the compiler may reload them at will instead
+// of holding them in registers across the K loop).
+constexpr int kGroupMax = 8;
+struct GroupGemm {
+  const double *Dr[kGroupMax];
+  const long long *dr_off[kGroupMax];
+  const double *X[kGroupMax];   // stage input of stage 0
+  const double *Xs[kGroupMax];  // stage input of stages 1..12
+  double *part[kGroupMax];
+  const SysDesc *d[kGroupMax];
+  const Ctl *ctl[kGroupMax];
+};
+
+// One member of a grouped launch (several batched propagators stepped with
+// one launch per kernel of the attempt): its descriptor, dipole copy,
+// stage buffers and step-control inputs, and where its workgroups start in
+// each grouped launch.
+struct GroupMember {
+  const SysDesc *d;
+  const double *Dr;          // row-major dipole copy
+  const long long *dr_off;
+  StageArgs sa;              // X, Xs, K, Y, part, E, red, ctl, NP (s unset)
+  double2 *X;                // sa.X, written by the apply step
+  double *part;              // sa.part, written by the coupling
+  const double *hs;
+  const double *Ftab;
+  const PulseDev *pulses;
+  const double *env;
+  LogEnt *log;
+  StepStore st;
+  int P, N, logcap, track;
+  int nblk_stage, wg_stage0;  // stage / finalize partial workgroups
+  int nblk_apply, wg_apply0;
+  int pulse0;                 // first k_finalize_g workgroup
+  int pad;
+};
+
 __global__ void k_relayout_gemv(const double *D, int sL, int sL1, int ncb,
                                 long long nelem, int job0, int njobs,
                                 double *out);
 __global__ void k_gemv_tile(const TileJob *jobs, int njobs, const double *Dg,
-                            const double2 *X, double2 *part, SysDesc d,
+                            const double2 *X, double2 *part, const SysDesc *d,
                             const Ctl *ctl);
 template <int WM, int WN>
 __global__ void k_gemm_lds(const GemmJob *jobs, const double *Dr,
                            const long long *dr_off, const double *X,
-                           double *part, SysDesc d, const Ctl *ctl);
+                           double *part, const SysDesc *d, const Ctl *ctl);
+template <int WM, int WN>
+__global__ void k_gemm_lds_g(const GemmJob *jobs, GroupGemm g, int s);
 template <int WM, int WN, int TK, int FM, int FN>
 __global__ void k_gemm_db(const GemmJob *jobs, const double *Dr,
                           const long long *dr_off, const double *X,
-                          double *part, SysDesc d, const Ctl *ctl);
+                          double *part, const SysDesc *d, const Ctl *ctl);
 template <int WM, int WN>
 __global__ void k_gemm_coupling(const GemmJob *jobs, const double *Dr,
                                 const long long *dr_off, const double *X,
-                                double *part, SysDesc d, const Ctl *ctl);
+                                double *part, const SysDesc *d, const Ctl *ctl);
 template <int SUB, bool LAST>
-__global__ void k_stage(StageArgs a, SysDesc d);
+__global__ void k_stage(StageArgs a, const SysDesc *d);
 template <bool LAST>
-__global__ void k_stage1(StageArgs a, SysDesc d);
+__global__ void k_stage_g(const GroupMember *m, int nm, int s);
+__global__ void k_finalize_g(const GroupMember *m, int nm);
+__global__ void k_apply_g(const GroupMember *m, int nm);
 template <bool LAST>
-__global__ void k_small_stage(StageArgs a, SysDesc d, const double *Dr,
+__global__ void k_stage1(StageArgs a, const SysDesc *d);
+template <bool LAST>
+__global__ void k_small_stage(StageArgs a, const SysDesc *d, const double *Dr,
                               const long long *dr_off, const int *dr_ld,
                               const double *DrT, const long long *drt_off,
                               const int *drt_ld);
@@ -89,8 +133,8 @@ __global__ void k_dot_partial(const double2 *a, const double2 *b, int N, int P,
                               double *red);
 __global__ void k_reduce_final(const double *red, int nblk, int P,
                                double *out);
-__global__ void k_adj_init(AdjArgs a, SysDesc d);
-__global__ void k_adj_stage(AdjArgs a, SysDesc d);
+__global__ void k_adj_init(AdjArgs a, const SysDesc *d);
+__global__ void k_adj_stage(AdjArgs a, const SysDesc *d);
 __global__ void k_weighted_pop(const double2 *X, const double *W, int N, int P,
                                double *out);
 // Ctl record of one RHS evaluation at constant field F (mode RHS).

@@ -93,6 +93,11 @@ _SIGS = [
                                       C.POINTER(_dp), C.c_double, C.c_double,
                                       C.c_int,
                                       C.POINTER(C.POINTER(MpciStats))]),
+    ("mpci_multi_adaptive_grouped", C.c_int, [C.c_int, C.POINTER(_vp),
+                                              C.POINTER(_dp), C.POINTER(_dp),
+                                              C.POINTER(_dp), C.c_double,
+                                              C.c_double, C.c_int,
+                                              C.POINTER(C.POINTER(MpciStats))]),
     ("mpci_prop_schedule", C.c_int, [_vp, _dp, C.c_int, _dp]),
     ("mpci_prop_populations", C.c_int, [_vp, C.c_int, _dp, _dp]),
     ("mpci_obs_weights", C.c_int, [C.c_int, _ip, C.POINTER(_dp), C.c_int,

@@ -282,11 +282,13 @@ class Propagator:
 
 
 def multi_adaptive(props, t0, t1, h0, atol, rtol, max_attempts=10_000_000,
-                   allow_limit=False):
+                   allow_limit=False, grouped=False):
     """Adaptive propagation of several propagators (different systems and
     batch sizes) concurrently on one device; equal to separate
     Propagator.prop_adaptive calls.  t0, t1, h0: one scalar or P-vector per
-    member.  Returns one list of per-pulse stats per member."""
+    member.  grouped: every member's attempt in the same launches
+    (batched members only); otherwise each member on its own stream.
+    Returns one list of per-pulse stats per member."""
     n = len(props)
     keep, st = [], []
 
@@ -303,8 +305,9 @@ def multi_adaptive(props, t0, t1, h0, atol, rtol, max_attempts=10_000_000,
         st.append((MpciStats * p.P)())
     sp = (C.POINTER(MpciStats) * n)(
         *[C.cast(s, C.POINTER(MpciStats)) for s in st])
-    rc = _lib.tdse_lib().mpci_multi_adaptive(n, hs, a0, a1, ah, atol, rtol,
-                                             max_attempts, sp)
+    fn = (_lib.tdse_lib().mpci_multi_adaptive_grouped if grouped
+          else _lib.tdse_lib().mpci_multi_adaptive)
+    rc = fn(n, hs, a0, a1, ah, atol, rtol, max_attempts, sp)
     if not (allow_limit and rc == _lib.MPCI_ERR_LIMIT):
         check(rc)
     return [list(s) for s in st]

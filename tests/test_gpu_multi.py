@@ -117,3 +117,57 @@ def test_multi_full_shapes(mpci):
         one.close()
     for pr in props:
         pr.close()
+
+
+def test_grouped_equals_separate_runs(mpci, members):
+    """Every member's attempt in the same launches (one interleaved GEMM job
+    list, one stage, finalize and apply launch per attempt): each member
+    equals its own separate run bit for bit, whatever tile the group uses
+    (each member keeps its own K split)."""
+    ms = members[:2]  # the batched members
+    props = [fresh(mpci, m) for m in ms]
+    st = mpci.multi_adaptive(props, [0.0] * 2, [m["t1"] for m in ms],
+                             [0.05] * 2, TOL, TOL, grouped=True)
+    for i, m in enumerate(ms):
+        one = fresh(mpci, m)
+        s1, _ = one.prop_adaptive(0.0, m["t1"], 0.05, TOL, TOL)
+        for p in range(m["P"]):
+            assert st[i][p].accepted == s1[p].accepted, (i, p)
+            assert st[i][p].rejected == s1[p].rejected, (i, p)
+            assert st[i][p].t == m["t1"][p]
+            assert np.array_equal(props[i].get_state(p), one.get_state(p)), \
+                (i, p)
+
+
+def test_grouped_refuses_single_pulse_member(mpci, members):
+    props = [fresh(mpci, m) for m in members]  # the third has P = 1
+    with pytest.raises(mpci.MpciError) as ei:
+        mpci.multi_adaptive(props, [0.0] * 3, [m["t1"] for m in members],
+                            [0.05] * 3, TOL, TOL, grouped=True)
+    assert ei.value.code == -5
+
+
+def test_grouped_full_shapes(mpci):
+    """The uneven multi-instance workload at its real shapes (32 pulses on
+    21 blocks of 998 states next to 32 on 4 blocks of 4000) in grouped
+    launches: three attempts each, every member equal to its own separate
+    run bit for bit."""
+    ms = [member(mpci, [998] * 21, "v", 51, 32, 61, 20),
+          member(mpci, [4000] * 4, "v", 20261202, 32, 62, 20)]
+    props = [fresh(mpci, m) for m in ms]
+    st = mpci.multi_adaptive(props, [0.0] * 2, [m["t1"] for m in ms],
+                             [0.05] * 2, TOL, TOL, max_attempts=3,
+                             allow_limit=True, grouped=True)
+    for i, m in enumerate(ms):
+        one = fresh(mpci, m)
+        s1, _ = one.prop_adaptive(0.0, m["t1"], 0.05, TOL, TOL,
+                                  max_attempts=3, allow_limit=True)
+        for p in range(m["P"]):
+            assert st[i][p].attempts == s1[p].attempts == 3, (i, p)
+            assert st[i][p].accepted == s1[p].accepted, (i, p)
+            assert st[i][p].t == s1[p].t, (i, p)
+            assert np.array_equal(props[i].get_state(p), one.get_state(p)), \
+                (i, p)
+        one.close()
+    for pr in props:
+        pr.close()
