@@ -104,3 +104,34 @@ def test_system_create_refuses_bad_arguments():
     for rc, msg, out in cases:
         assert rc == -1 and msg and out is None, (rc, msg)
     assert "too large" in cases[-1][1]
+
+
+def test_new_entries_refuse_bad_arguments_before_device_calls():
+    """mpci_system_rhs_noalloc refuses a NULL system or state, and
+    mpci_multi_adaptive_grouped refuses an empty group, more than eight
+    members and NULL arrays, each with MPCI_ERR_ARG and a message, before
+    any pointer is read or the device touched."""
+    from mpci import _lib
+    lib = ctypes.CDLL(_lib.TDSE_LIB)
+    lib.mpci_last_error.restype = ctypes.c_char_p
+    na = lib.mpci_system_rhs_noalloc
+    na.restype = ctypes.c_int
+    na.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                   ctypes.c_void_p, ctypes.c_void_p]
+    buf = (ctypes.c_double * 4)()
+    ok = ctypes.addressof(buf)
+    for args in [(None, 0.01, ok, ok, None), (ok, 0.01, None, ok, None),
+                 (ok, 0.01, ok, None, None)]:
+        assert na(*args) == -1
+        assert b"NULL" in lib.mpci_last_error()
+    g = lib.mpci_multi_adaptive_grouped
+    g.restype = ctypes.c_int
+    g.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 4 + [
+        ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    arr = (ctypes.c_void_p * 9)(*([None] * 9))
+    a = ctypes.addressof(arr)
+    for n, p in [(0, a), (9, a), (2, None)]:
+        assert g(n, p, a, a, a, 1e-10, 1e-10, 10, None) == -1
+        assert b"1 to 8 members" in lib.mpci_last_error()
+    assert g(2, a, a, a, a, 1e-10, 1e-10, 10, None) == -1
+    assert b"NULL member" in lib.mpci_last_error()
