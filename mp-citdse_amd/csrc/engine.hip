@@ -131,6 +131,7 @@ struct mpci_prop {
   bool small = false;
   int nblk_small = 0;
   DevBuf<double2> X, Xs, K, Y, part;
+  DevBuf<double2> xstage;  // one pulse's contiguous state (set/get_state)
   DevBuf<double2> Xs2;  // second stage-input buffer of the fused path
   DevBuf<double> red;
   DevBuf<Ctl> ctl;
@@ -526,6 +527,18 @@ static int ensure_transposed(mpci_system *sys) {
 // GEMM jobs: for every pair and direction, output panels of 16*fm*wm rows
 // x 16*fn*wn real columns (wm x wn waves, each fm x fn MFMA blocks of
 // 16 x 16), each over one of `ks` equal K ranges.
+//
+// Sweep start (kr, used by the direct-to-LDS kernel): both directions of a
+// pair read the same D_L, the D x_{L+1} panels along its rows and the
+// D^T x_L panels down its columns, and every panel of a launch runs at
+// once.  A panel whose 128-row group starts `off` rows into the other
+// direction's K range starts its own K sweep `off` before its range's
+// start (wrapping), so element (i, j) is read by its row panel and by its
+// column panel at the same point of their sweeps, to within 128 / 16
+// chunks: the second read finds the tile still in the device's caches
+// instead of streaming D_L from HBM a second time.  The grouping is fixed
+// at 128 rows (the tallest panel), so the start, and with it the summation
+// order, depends on the K split alone, not on the tile chosen.
 static std::vector<GemmJob> gemm_job_list(const mpci_prop *pr, int wm, int wn,
                                           int fm, int fn, int ks, int member) {
   const mpci_system *sys = pr->sys;
@@ -536,6 +549,8 @@ static std::vector<GemmJob> gemm_job_list(const mpci_prop *pr, int wm, int wn,
       const int M = dir == 0 ? sys->s[L] : sys->s[L + 1];
       const int K = dir == 0 ? sys->s[L + 1] : sys->s[L];
       const int kchunk = ((K + ks - 1) / ks + kGmTK - 1) / kGmTK * kGmTK;
+      // the other direction's K ranges split this one's M
+      const int kchunk_o = ((M + ks - 1) / ks + kGmTK - 1) / kGmTK * kGmTK;
       for (int n0 = 0; n0 < P2; n0 += TN)
         for (int q = 0; q < ks; ++q)
           for (int m0 = 0; m0 < M; m0 += TM) {
@@ -549,6 +564,12 @@ static std::vector<GemmJob> gemm_job_list(const mpci_prop *pr, int wm, int wn,
             j.q = q;
             j.ld = sys->dr_ld[L];
             j.member = member;
+            // multiples of kGmTK: off (128 m / kchunk_o), lpad; so is kr,
+            // and kr <= lpad - kGmTK < len
+            const int off = (m0 & ~127) % kchunk_o;
+            const int len = j.k1 - j.k0;
+            const int lpad = (len + kGmTK - 1) / kGmTK * kGmTK;
+            j.kr = len > 0 ? (lpad - off % lpad) % lpad : 0;
             out.push_back(j);
           }
     }
@@ -880,11 +901,16 @@ extern "C" int mpci_prop_set_state(mpci_prop *pr, int p, const double *ct) {
     return fail(MPCI_ERR_ARG, "mpci_prop_set_state: bad argument");
   if (pr->P == 1) {
     return h2d(pr, pr->X.p, ct, sizeof(double2) * pr->sys->N);
-    return MPCI_OK;
   }
-  HIPCHK(hipMemcpy2DAsync(pr->X.p + p, sizeof(double2) * pr->P, ct,
-                          sizeof(double2), sizeof(double2), pr->sys->N,
-                          hipMemcpyHostToDevice, pr->stream));
+  // one contiguous copy, then a strided kernel copy into the batch layout
+  // (a 2-D copy of N 16-byte rows from host memory moves row by row)
+  const int N = pr->sys->N;
+  if (!pr->xstage.p) TRY(pr->xstage.alloc(N));
+  HIPCHK(hipMemcpyAsync(pr->xstage.p, ct, sizeof(double2) * N,
+                        hipMemcpyHostToDevice, pr->stream));
+  hipLaunchKernelGGL(k_copy_strided, dim3((N + 255) / 256), dim3(256), 0,
+                     pr->stream, pr->X.p + p, pr->P, pr->xstage.p, 1, N);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(pr->stream));
   return MPCI_OK;
 }
@@ -895,11 +921,14 @@ extern "C" int mpci_prop_get_state(mpci_prop *pr, int p, double *ct) {
   HIPCHK(hipStreamSynchronize(pr->stream));
   if (pr->P == 1) {
     return d2h(pr, ct, pr->X.p, sizeof(double2) * pr->sys->N);
-    return MPCI_OK;
   }
-  HIPCHK(hipMemcpy2DAsync(ct, sizeof(double2), pr->X.p + p,
-                          sizeof(double2) * pr->P, sizeof(double2), pr->sys->N,
-                          hipMemcpyDeviceToHost, pr->stream));
+  const int N = pr->sys->N;
+  if (!pr->xstage.p) TRY(pr->xstage.alloc(N));
+  hipLaunchKernelGGL(k_copy_strided, dim3((N + 255) / 256), dim3(256), 0,
+                     pr->stream, pr->xstage.p, 1, pr->X.p + p, pr->P, N);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(ct, pr->xstage.p, sizeof(double2) * N,
+                        hipMemcpyDeviceToHost, pr->stream));
   HIPCHK(hipStreamSynchronize(pr->stream));
   return MPCI_OK;
 }

@@ -703,6 +703,7 @@ __device__ __forceinline__ void gemm_lds_body(
   constexpr int NW = WM * WN;
   constexpr int NIA = TM * TK / 128, NIB = TK * TN / 128;  // 1 KiB loads
   static_assert(NIA % NW == 0 && NIB % NW == 0, "loads must split evenly");
+  static_assert(kGmTK % TK == 0, "sweep starts fall on chunk boundaries");
   constexpr int LA = NIA / NW, LB = NIB / NW;  // per wave
   constexpr unsigned kOut = 0x80000000u;       // beyond any buffer range
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -744,21 +745,24 @@ __device__ __forceinline__ void gemm_lds_body(
   }
   const unsigned a_step = DIR == 0 ? TK * 8u : (unsigned)(TK * ld * 8);
   const unsigned b_step = (unsigned)(TK * P2 * 8);
-  // chunk c (k0 = jb.k0 + c TK)
+  const int nch = (jb.k1 - jb.k0 + TK - 1) / TK;
+  const int rot = jb.kr / TK;  // < nch: kr < k1 - k0, a multiple of TK
+  // sweep step c loads chunk cc = (c + rot) mod nch (k = jb.k0 + cc TK)
   auto issue = [&](int c, double *As, double *Bs) {
+    const int cc = c + rot < nch ? c + rot : c + rot - nch;
 #pragma unroll
     for (int u = 0; u < LA; ++u) {
-      const bool ok = a_in[u] && a_k[u] + c * TK < jb.k1;
+      const bool ok = a_in[u] && a_k[u] + cc * TK < jb.k1;
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
           rsA, (lds_void_t *)(As + (w * LA + u) * 128), 16,
-          ok ? a_off[u] + c * a_step : kOut, 0, 0, 0);
+          ok ? a_off[u] + cc * a_step : kOut, 0, 0, 0);
     }
 #pragma unroll
     for (int u = 0; u < LB; ++u) {
-      const bool ok = b_in[u] && b_k[u] + c * TK < jb.k1;
+      const bool ok = b_in[u] && b_k[u] + cc * TK < jb.k1;
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
           rsB, (lds_void_t *)(Bs + (w * LB + u) * 128), 16,
-          ok ? b_off[u] + c * b_step : kOut, 0, 0, 0);
+          ok ? b_off[u] + cc * b_step : kOut, 0, 0, 0);
     }
   };
 
@@ -810,7 +814,6 @@ __device__ __forceinline__ void gemm_lds_body(
     }
   };
 
-  const int nch = (jb.k1 - jb.k0 + TK - 1) / TK;
   if (nch > 0) issue(0, As0, Bs0);
   __syncthreads();  // chunk 0 landed (each wave waits for its own loads)
   int c = 0;
@@ -1871,6 +1874,15 @@ __global__ void k_rhs_ctl(Ctl *ctl, double F) {
     for (int s = 0; s < kStages; ++s) c.Fs[s] = F;
     ctl[0] = c;
   }
+}
+
+// One pulse's state into or out of the pulse-interleaved batch layout
+// (element i of pulse p at i P + p): dst[i dst_stride] = src[i src_stride].
+__global__ __launch_bounds__(256) void k_copy_strided(
+    double2 *__restrict__ dst, int dst_stride, const double2 *__restrict__ src,
+    int src_stride, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[(long long)i * dst_stride] = src[(long long)i * src_stride];
 }
 
 // ---------------------------------------------------------------------

@@ -139,6 +139,8 @@ __global__ void k_weighted_pop(const double2 *X, const double *W, int N, int P,
                                double *out);
 // Ctl record of one RHS evaluation at constant field F (mode RHS).
 __global__ void k_rhs_ctl(Ctl *ctl, double F);
+__global__ void k_copy_strided(double2 *dst, int dst_stride,
+                               const double2 *src, int src_stride, int n);
 __global__ void k_probe_read(const double2 *a, long long n, double *out);
 __global__ void k_probe_mfma(double *out, int iters);
 
