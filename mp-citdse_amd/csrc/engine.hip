@@ -538,11 +538,18 @@ static int ensure_transposed(mpci_system *sys) {
 // chunks: the second read finds the tile still in the device's caches
 // instead of streaming D_L from HBM a second time.  The grouping is fixed
 // at 128 rows (the tallest panel), so the start, and with it the summation
-// order, depends on the K split alone, not on the tile chosen.
+// order, depends on the K split and P alone, not on the tile chosen.
+// Applied up to 32 pulses, where the dipole stream bounds the coupling.
+// Measured per evaluation on He 4 x 4000, same tile without -> with:
+// P = 16 164.6 -> 146.3 us (4 x 1 waves, K split 4), P = 32 243.0 ->
+// 223.3 us (4 x 2, K split 4).  At P = 64 the MFMA work bounds it, and X
+// rows read at scattered K positions cost more than the saved dipole
+// reads (autotuned tile: 400.9 -> 404.6 us).
 static std::vector<GemmJob> gemm_job_list(const mpci_prop *pr, int wm, int wn,
                                           int fm, int fn, int ks, int member) {
   const mpci_system *sys = pr->sys;
   const int TM = 16 * fm * wm, TN = 16 * fn * wn, P2 = 2 * pr->P;
+  const bool align = pr->P <= 32;
   std::vector<GemmJob> out;
   for (int L = 0; L + 1 < sys->nb; ++L)
     for (int dir = 0; dir < 2; ++dir) {
@@ -569,7 +576,7 @@ static std::vector<GemmJob> gemm_job_list(const mpci_prop *pr, int wm, int wn,
             const int off = (m0 & ~127) % kchunk_o;
             const int len = j.k1 - j.k0;
             const int lpad = (len + kGmTK - 1) / kGmTK * kGmTK;
-            j.kr = len > 0 ? (lpad - off % lpad) % lpad : 0;
+            j.kr = (align && len > 0) ? (lpad - off % lpad) % lpad : 0;
             out.push_back(j);
           }
     }
@@ -999,7 +1006,8 @@ static int enqueue_tile(mpci_prop *pr, const double2 *xin, int s) {
                           dim3((unsigned)std::min(njobs, sys->gv_grid)),
                           dim3(kGvThreads), 0, pr->stream, e0, e1, 0,
                           sys->djobs.p, njobs, sys->Dg.p, xin, pr->part.p,
-                          pr->ddesc.p, pr->ctl.p);
+                          pr->ddesc.p, pr->ctl.p,
+                          s & 1);  // stages alternate sweep directions
     HIPCHK(hipGetLastError());
   } else {
     TRY(launch_gemm(pr, xin, e0, e1));
@@ -1336,7 +1344,7 @@ extern "C" int mpci_system_rhs_noalloc(const mpci_system *sys, double F,
     hipLaunchKernelGGL(k_gemv_tile,
                        dim3((unsigned)std::min(njobs, sys->gv_grid)),
                        dim3(kGvThreads), 0, st, sys->djobs.p, njobs, sys->Dg.p,
-                       xv, part, sys->ddesc1.p, ctl);
+                       xv, part, sys->ddesc1.p, ctl, 0);
     HIPCHK(hipGetLastError());
   }
   StageArgs a{};

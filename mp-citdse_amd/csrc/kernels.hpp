@@ -89,7 +89,7 @@ __global__ void k_relayout_gemv(const double *D, int sL, int sL1, int ncb,
                                 double *out);
 __global__ void k_gemv_tile(const TileJob *jobs, int njobs, const double *Dg,
                             const double2 *X, double2 *part, const SysDesc *d,
-                            const Ctl *ctl);
+                            const Ctl *ctl, int rev);
 template <int WM, int WN>
 __global__ void k_gemm_lds(const GemmJob *jobs, const double *Dr,
                            const long long *dr_off, const double *X,
