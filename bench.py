@@ -647,6 +647,14 @@ def main():
             b["mfma_probe"] = ceil["mfma_probe"]
         out["batch64"] = {k: (round(v, 4) if isinstance(v, float) else v)
                           for k, v in b.items()}
+        # 16 pulses: the adjoint workload's batch size, where the dipole
+        # stream rather than the MFMA work bounds the coupling
+        log("bench: batch-16 ...")
+        b = run_batch(mpci, synth, sysg, P=16)
+        if b["mfma_tflops"] is not None:
+            b["mfma_frac"] = b["mfma_tflops"] / 78.6
+        out["batch16"] = {k: (round(v, 4) if isinstance(v, float) else v)
+                          for k, v in b.items()}
     if not args.no_multi:
         log("bench: multi-instance 32 H (21 x 998) + 32 He ...")
         mres = run_multi(mpci, synth, sysg)

@@ -539,12 +539,11 @@ static int ensure_transposed(mpci_system *sys) {
 // instead of streaming D_L from HBM a second time.  The grouping is fixed
 // at 128 rows (the tallest panel), so the start, and with it the summation
 // order, depends on the K split and P alone, not on the tile chosen.
-// Applied up to 32 pulses, where the dipole stream bounds the coupling.
-// Measured per evaluation on He 4 x 4000, same tile without -> with:
-// P = 16 164.6 -> 146.3 us (4 x 1 waves, K split 4), P = 32 243.0 ->
-// 223.3 us (4 x 2, K split 4).  At P = 64 the MFMA work bounds it, and X
-// rows read at scattered K positions cost more than the saved dipole
-// reads (autotuned tile: 400.9 -> 404.6 us).
+// Applied up to 32 pulses.  Measured per evaluation on He 4 x 4000
+// without -> with: P = 16 about 144.8 -> 141.2 us, P = 32 unchanged
+// (212 us).  At P = 64 the MFMA work bounds the coupling, and X rows read
+// at scattered K positions cost more than the saved dipole reads
+// (400.9 -> 404.6 us), so the offsets stay zero there.
 static std::vector<GemmJob> gemm_job_list(const mpci_prop *pr, int wm, int wn,
                                           int fm, int fn, int ks, int member) {
   const mpci_system *sys = pr->sys;
