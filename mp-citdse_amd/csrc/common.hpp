@@ -58,9 +58,6 @@ struct GemmJob {
   int q;        // partial slot (K-split index)
   int ld;       // row stride of D_L in the row-major copy (even)
   int member;   // grouped launches: the member the panel belongs to
-  int kr;       // direct-to-LDS kernel: K offset from k0 (a multiple of
-                // kGmTK, below k1 - k0) where the panel's K sweep starts;
-                // it wraps to k0 after k1
 };
 
 // Per-propagator description of the blocks and the partial-sum buffers.

@@ -527,36 +527,17 @@ static int ensure_transposed(mpci_system *sys) {
 // GEMM jobs: for every pair and direction, output panels of 16*fm*wm rows
 // x 16*fn*wn real columns (wm x wn waves, each fm x fn MFMA blocks of
 // 16 x 16), each over one of `ks` equal K ranges.
-//
-// Sweep start (kr, used by the direct-to-LDS kernel): both directions of a
-// pair read the same D_L, the D x_{L+1} panels along its rows and the
-// D^T x_L panels down its columns, and every panel of a launch runs at
-// once.  A panel whose 128-row group starts `off` rows into the other
-// direction's K range starts its own K sweep `off` before its range's
-// start (wrapping), so element (i, j) is read by its row panel and by its
-// column panel at the same point of their sweeps, to within 128 / 16
-// chunks: the second read finds the tile still in the device's caches
-// instead of streaming D_L from HBM a second time.  The grouping is fixed
-// at 128 rows (the tallest panel), so the start, and with it the summation
-// order, depends on the K split and P alone, not on the tile chosen.
-// Applied up to 32 pulses.  Measured per evaluation on He 4 x 4000
-// without -> with: P = 16 about 144.8 -> 141.2 us, P = 32 unchanged
-// (212 us).  At P = 64 the MFMA work bounds the coupling, and X rows read
-// at scattered K positions cost more than the saved dipole reads
-// (400.9 -> 404.6 us), so the offsets stay zero there.
+
 static std::vector<GemmJob> gemm_job_list(const mpci_prop *pr, int wm, int wn,
                                           int fm, int fn, int ks, int member) {
   const mpci_system *sys = pr->sys;
   const int TM = 16 * fm * wm, TN = 16 * fn * wn, P2 = 2 * pr->P;
-  const bool align = pr->P <= 32;
   std::vector<GemmJob> out;
   for (int L = 0; L + 1 < sys->nb; ++L)
     for (int dir = 0; dir < 2; ++dir) {
       const int M = dir == 0 ? sys->s[L] : sys->s[L + 1];
       const int K = dir == 0 ? sys->s[L + 1] : sys->s[L];
       const int kchunk = ((K + ks - 1) / ks + kGmTK - 1) / kGmTK * kGmTK;
-      // the other direction's K ranges split this one's M
-      const int kchunk_o = ((M + ks - 1) / ks + kGmTK - 1) / kGmTK * kGmTK;
       for (int n0 = 0; n0 < P2; n0 += TN)
         for (int q = 0; q < ks; ++q)
           for (int m0 = 0; m0 < M; m0 += TM) {
@@ -570,12 +551,6 @@ static std::vector<GemmJob> gemm_job_list(const mpci_prop *pr, int wm, int wn,
             j.q = q;
             j.ld = sys->dr_ld[L];
             j.member = member;
-            // multiples of kGmTK: off (128 m / kchunk_o), lpad; so is kr,
-            // and kr <= lpad - kGmTK < len
-            const int off = (m0 & ~127) % kchunk_o;
-            const int len = j.k1 - j.k0;
-            const int lpad = (len + kGmTK - 1) / kGmTK * kGmTK;
-            j.kr = (align && len > 0) ? (lpad - off % lpad) % lpad : 0;
             out.push_back(j);
           }
     }

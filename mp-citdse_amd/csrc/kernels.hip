@@ -717,7 +717,6 @@ __device__ __forceinline__ void gemm_lds_body(
   constexpr int NW = WM * WN;
   constexpr int NIA = TM * TK / 128, NIB = TK * TN / 128;  // 1 KiB loads
   static_assert(NIA % NW == 0 && NIB % NW == 0, "loads must split evenly");
-  static_assert(kGmTK % TK == 0, "sweep starts fall on chunk boundaries");
   constexpr int LA = NIA / NW, LB = NIB / NW;  // per wave
   constexpr unsigned kOut = 0x80000000u;       // beyond any buffer range
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -759,24 +758,21 @@ __device__ __forceinline__ void gemm_lds_body(
   }
   const unsigned a_step = DIR == 0 ? TK * 8u : (unsigned)(TK * ld * 8);
   const unsigned b_step = (unsigned)(TK * P2 * 8);
-  const int nch = (jb.k1 - jb.k0 + TK - 1) / TK;
-  const int rot = jb.kr / TK;  // < nch: kr < k1 - k0, a multiple of TK
-  // sweep step c loads chunk cc = (c + rot) mod nch (k = jb.k0 + cc TK)
+  // chunk c (k0 = jb.k0 + c TK)
   auto issue = [&](int c, double *As, double *Bs) {
-    const int cc = c + rot < nch ? c + rot : c + rot - nch;
 #pragma unroll
     for (int u = 0; u < LA; ++u) {
-      const bool ok = a_in[u] && a_k[u] + cc * TK < jb.k1;
+      const bool ok = a_in[u] && a_k[u] + c * TK < jb.k1;
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
           rsA, (lds_void_t *)(As + (w * LA + u) * 128), 16,
-          ok ? a_off[u] + cc * a_step : kOut, 0, 0, 0);
+          ok ? a_off[u] + c * a_step : kOut, 0, 0, 0);
     }
 #pragma unroll
     for (int u = 0; u < LB; ++u) {
-      const bool ok = b_in[u] && b_k[u] + cc * TK < jb.k1;
+      const bool ok = b_in[u] && b_k[u] + c * TK < jb.k1;
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
           rsB, (lds_void_t *)(Bs + (w * LB + u) * 128), 16,
-          ok ? b_off[u] + cc * b_step : kOut, 0, 0, 0);
+          ok ? b_off[u] + c * b_step : kOut, 0, 0, 0);
     }
   };
 
@@ -828,6 +824,7 @@ __device__ __forceinline__ void gemm_lds_body(
     }
   };
 
+  const int nch = (jb.k1 - jb.k0 + TK - 1) / TK;
   if (nch > 0) issue(0, As0, Bs0);
   __syncthreads();  // chunk 0 landed (each wave waits for its own loads)
   int c = 0;
@@ -837,7 +834,7 @@ __device__ __forceinline__ void gemm_lds_body(
     // every MFMA of the chunk is issued before the wave waits for the
     // next chunk's loads at the barrier, so they run while those loads are
     // in flight (left to itself, the scheduler sinks most of them below the
-    // barrier: 16 pulses, 139 -> 131 us per evaluation)
+    // barrier: 16 pulses, 143.8 -> 134.7 us per evaluation)
     __builtin_amdgcn_sched_barrier(0);
     __syncthreads();  // buffer 0 read by all waves; chunk c + 1 landed
     if (c + 2 < nch) issue(c + 2, As0, Bs0);
