@@ -203,6 +203,30 @@ def test_every_gemm_candidate(mpci, oracle, P):
     assert len(seen) == mpci.Propagator.gemm_candidates()
 
 
+@pytest.mark.parametrize("P", [3, 40])
+def test_candidates_with_equal_k_split_agree_bitwise(mpci, oracle, P):
+    """Every batched kernel sums a panel's K range in the same order
+    (increasing k, four per MFMA), so tiles and kernels that share a K
+    split give the same bits.  A grouped launch (always the direct-to-LDS
+    kernel, the group's tile, each member's own K split) relies on this to
+    reproduce each member's own run, whichever tile its autotuner chose."""
+    g, o = build(mpci, oracle, [70, 130, 45, 90], "v", seed=P + 300)
+    prop = mpci.Propagator(g, P=P)
+    rng = np.random.default_rng(P + 7)
+    x = rng.standard_normal((P, 2 * g.N)).reshape(-1)
+    F = rng.uniform(-1, 1, P)
+    by_split = {}
+    for i in range(mpci.Propagator.gemm_candidates()):
+        cfg = prop.set_gemm_candidate(i)
+        by_split.setdefault(cfg["ksplit"], []).append(
+            (cfg, prop.rhs(F, x).copy()))
+    assert len(by_split) >= 2
+    for ks, runs in by_split.items():
+        cfg0, ref = runs[0]
+        for cfg, got in runs[1:]:
+            assert np.array_equal(got, ref), (ks, cfg0, cfg)
+
+
 def test_pinned_candidate_in_captured_steps(mpci, oracle):
     """Pinning a tile after steps were captured rebuilds the captured step
     (the job list and partial buffers are reallocated) and gives the same
