@@ -58,7 +58,16 @@ struct GemmJob {
   int q;        // partial slot (K-split index)
   int ld;       // row stride of D_L in the row-major copy (even)
   int member;   // grouped launches: the member the panel belongs to
+  int qt;       // k_gemm_pair: the D^T x_L partial slot (strip index); q
+                // is then the D x_{L+1} slot (column-range index)
 };
+
+// k_gemm_pair geometry: a workgroup of 8 waves takes a strip of kPrR rows
+// of D_L over a range of its columns, 16 columns (one LDS chunk) at a time,
+// for one 32-column panel of X.
+constexpr int kPrR = 256;
+constexpr int kPrTK = 16;
+constexpr int kPrTN = 32;
 
 // Per-propagator description of the blocks and the partial-sum buffers.
 struct SysDesc {

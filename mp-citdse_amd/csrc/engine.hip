@@ -94,6 +94,7 @@ struct mpci_system {
   long long S = 0;             // sum_L s_L s_{L+1} (dipole elements)
   int device = 0;
   int gv_grid = 512;  // persistent GEMV workgroups: 2 per compute unit
+  int ncu = 256;      // compute units of the device
   // row-major copy of every D_L for the batched (GEMM) form, built on the
   // first propagator with P >= 2 from the host blocks kept until then
   DevBuf<double> Dr;
@@ -372,8 +373,10 @@ extern "C" int mpci_system_create(int nb, const int *s, char gauge,
   (void)hipGetDevice(&sys->device);
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
-                            sys->device) == hipSuccess && ncu > 0)
+                            sys->device) == hipSuccess && ncu > 0) {
     sys->gv_grid = 2 * ncu;
+    sys->ncu = ncu;
+  }
   int rc = build_system(sys, E, D);
   if (rc != MPCI_OK) {
     delete sys;
@@ -435,8 +438,12 @@ static int stage1_blocks(const mpci_system *sys) {
 
 // Fills d for P pulses (K split ks in the batched form) and returns the
 // number of double2 partials the layout needs.
-static long long fill_desc(const mpci_system *sys, int P, int ks,
-                           SysDesc &d) {
+// Partial-slot layout: pair L keeps nrow[L] partial vectors of D_L x_{L+1}
+// (length s_L) and ncol[L] of D_L^T x_L (length s_{L+1}); returns the
+// double2 count.
+static long long fill_desc_counts(const mpci_system *sys, int P,
+                                  const int *nrow, const int *ncol,
+                                  SysDesc &d) {
   std::memset(&d, 0, sizeof(d));
   d.nb = sys->nb;
   d.N = sys->N;
@@ -445,8 +452,8 @@ static long long fill_desc(const mpci_system *sys, int P, int ks,
   for (int L = 0; L <= sys->nb; ++L) d.off[L] = sys->off[L];
   long long o = 0;
   for (int L = 0; L + 1 < sys->nb; ++L) {
-    d.nrow_part[L] = P == 1 ? sys->ncb[L] : ks;
-    d.ncol_part[L] = P == 1 ? sys->nrb[L] : ks;
+    d.nrow_part[L] = nrow[L];
+    d.ncol_part[L] = ncol[L];
     d.row_part_off[L] = o;
     o += (long long)d.nrow_part[L] * sys->s[L] * P;
     d.col_part_off[L] = o;
@@ -455,12 +462,67 @@ static long long fill_desc(const mpci_system *sys, int P, int ks,
   return o;
 }
 
-static int init_desc(mpci_prop *pr, int ks) {
-  const long long o = fill_desc(pr->sys, pr->P, ks, pr->desc);
+// One pulse: one slot per GEMV unit column / row; batched: ks K-split
+// slots per direction.
+static long long fill_desc(const mpci_system *sys, int P, int ks,
+                           SysDesc &d) {
+  std::vector<int> nr(std::max(sys->nb - 1, 1)), nc(nr.size());
+  for (int L = 0; L + 1 < sys->nb; ++L) {
+    nr[L] = P == 1 ? sys->ncb[L] : ks;
+    nc[L] = P == 1 ? sys->nrb[L] : ks;
+  }
+  return fill_desc_counts(sys, P, nr.data(), nc.data(), d);
+}
+
+static int upload_desc(mpci_prop *pr, long long o) {
   TRY(pr->part.alloc(std::max<long long>(o, 1)));
   if (!pr->ddesc.p) TRY(pr->ddesc.alloc(1));
   TRY(h2d(pr, pr->ddesc.p, &pr->desc, sizeof(SysDesc)));
   return MPCI_OK;
+}
+
+static int init_desc(mpci_prop *pr, int ks) {
+  return upload_desc(pr, fill_desc(pr->sys, pr->P, ks, pr->desc));
+}
+
+// k_gemm_pair's split of pair L: strips of kPrR rows (nS[L], one D^T x_L
+// slot each) and column ranges of crange[L] columns (nC[L], one D x_{L+1}
+// slot each), the ranges sized so that the jobs of all pairs and 32-column
+// panels number about f x (compute units): one workgroup per CU holds the
+// kernel's LDS.
+static void pair_split(const mpci_prop *pr, int f, std::vector<int> &nS,
+                       std::vector<int> &nC, std::vector<int> &crange) {
+  const mpci_system *sys = pr->sys;
+  const int np = std::max(sys->nb - 1, 1);
+  nS.assign(np, 0);
+  nC.assign(np, 0);
+  crange.assign(np, 0);
+  const int panels = (2 * pr->P + kPrTN - 1) / kPrTN;
+  long long strips = 0;
+  for (int L = 0; L + 1 < sys->nb; ++L) {
+    nS[L] = (sys->s[L] + kPrR - 1) / kPrR;
+    strips += nS[L];
+  }
+  const long long target = (long long)f * sys->ncu / std::max(panels, 1);
+  const int nc = (int)std::max<long long>(1, target / std::max(strips, 1LL));
+  for (int L = 0; L + 1 < sys->nb; ++L) {
+    const int cols = sys->s[L + 1];
+    const int cr = ((cols + nc - 1) / nc + kPrTK - 1) / kPrTK * kPrTK;
+    crange[L] = cr;
+    nC[L] = (cols + cr - 1) / cr;  // every range holds columns
+  }
+}
+
+static int init_desc_pair(mpci_prop *pr, int f) {
+  std::vector<int> nS, nC, cr;
+  pair_split(pr, f, nS, nC, cr);
+  return upload_desc(pr, fill_desc_counts(pr->sys, pr->P, nC.data(),
+                                          nS.data(), pr->desc));
+}
+
+// the partial layout of candidate kind (3: k_gemm_pair, ks its f)
+static int init_desc_kind(mpci_prop *pr, int kind, int ks) {
+  return kind == 3 ? init_desc_pair(pr, ks) : init_desc(pr, ks);
 }
 
 // Row-major copy of every D_L with an even row stride ld_L (the padding
@@ -557,9 +619,38 @@ static std::vector<GemmJob> gemm_job_list(const mpci_prop *pr, int wm, int wn,
   return out;
 }
 
+// k_gemm_pair's jobs: per 32-column panel of X, pair, strip and column
+// range (pair_split).
+static std::vector<GemmJob> pair_job_list(const mpci_prop *pr, int f,
+                                          int member) {
+  const mpci_system *sys = pr->sys;
+  std::vector<int> nS, nC, cr;
+  pair_split(pr, f, nS, nC, cr);
+  std::vector<GemmJob> out;
+  for (int n0 = 0; n0 < 2 * pr->P; n0 += kPrTN)
+    for (int L = 0; L + 1 < sys->nb; ++L)
+      for (int st = 0; st < nS[L]; ++st)
+        for (int ci = 0; ci < nC[L]; ++ci) {
+          GemmJob j{};
+          j.L = L;
+          j.dir = 2;
+          j.m0 = st * kPrR;
+          j.n0 = n0;
+          j.k0 = ci * cr[L];
+          j.k1 = std::min(sys->s[L + 1], (ci + 1) * cr[L]);
+          j.q = ci;
+          j.qt = st;
+          j.ld = sys->dr_ld[L];
+          j.member = member;
+          out.push_back(j);
+        }
+  return out;
+}
+
 static int build_gemm_jobs(mpci_prop *pr, int kind, int wm, int wn, int fm,
                            int fn, int tk, int ks) {
-  pr->gjobs = gemm_job_list(pr, wm, wn, fm, fn, ks, 0);
+  pr->gjobs = kind == 3 ? pair_job_list(pr, ks, 0)
+                        : gemm_job_list(pr, wm, wn, fm, fn, ks, 0);
   TRY(pr->dgjobs.alloc(pr->gjobs.size()));
   if (!pr->gjobs.empty())
     TRY(h2d(pr, pr->dgjobs.p, pr->gjobs.data(),
@@ -597,7 +688,11 @@ static int launch_gemm(mpci_prop *pr, const double2 *xin,
                         sys->dr_off.p, X, part, pr->ddesc.p, pr->ctl.p)
   const int wm = pr->gm_wm, wn = pr->gm_wn, tk = pr->gm_tk;
   const int f = pr->gm_fm * 10 + pr->gm_fn;
-  if (pr->gm_kind == 2) {
+  if (pr->gm_kind == 3) {
+    hipExtLaunchKernelGGL(k_gemm_pair, grid, dim3(512), 0, pr->stream, e0, e1,
+                          0, pr->dgjobs.p, sys->Dr.p, sys->dr_off.p, X, part,
+                          pr->ddesc.p, pr->ctl.p);
+  } else if (pr->gm_kind == 2) {
     if (wm == 2 && wn == 1)
       MPCI_GEMM_LDS(2, 1);
     else if (wm == 4 && wn == 1)
@@ -667,7 +762,9 @@ static int launch_gemm(mpci_prop *pr, const double2 *xin,
 // buffer, K chunks of 32, 2 x 2 blocks per wave); kind 1: k_gemm_db
 // (double-buffered, 16-byte loads, K chunks of tk, fm x fn blocks per wave);
 // kind 2: k_gemm_lds (double-buffered direct-to-LDS loads, K chunks of 16,
-// 2 x 2 blocks per wave).
+// 2 x 2 blocks per wave); kind 3: k_gemm_pair (both directions from one
+// read of each dipole block, 8 waves; ks is the number of workgroups per
+// compute unit its split aims at).
 struct GemmCand {
   int kind, wm, wn, fm, fn, tk, ks;
 };
@@ -693,7 +790,7 @@ static const GemmCand kGemmCands[] = {
     {2, 2, 2, 2, 2, 16, 4}, {2, 4, 2, 2, 2, 16, 1}, {2, 4, 2, 2, 2, 16, 2},
     {2, 4, 2, 2, 2, 16, 4}, {2, 2, 4, 2, 2, 16, 1}, {2, 2, 4, 2, 2, 16, 2},
     {2, 2, 4, 2, 2, 16, 4}, {2, 4, 4, 2, 2, 16, 1}, {2, 4, 4, 2, 2, 16, 2},
-    {2, 4, 4, 2, 2, 16, 4}};
+    {2, 4, 4, 2, 2, 16, 4}, {3, 8, 1, 2, 2, 16, 1}, {3, 8, 1, 2, 2, 16, 2}};
 
 // k_gemm_lds addresses its operands through 32-bit buffer offsets: every
 // D_L copy and the X slab of the propagator must stay below 2 GiB.
@@ -710,7 +807,7 @@ static bool gemm_lds_fits(const mpci_prop *pr) {
 // between HIP events on the propagator's stream (all pulses live).
 static int time_gemm_cand(mpci_prop *pr, const GemmCand &cd, hipEvent_t e0,
                           hipEvent_t e1, float *ms) {
-  TRY(init_desc(pr, cd.ks));
+  TRY(init_desc_kind(pr, cd.kind, cd.ks));
   TRY(build_gemm_jobs(pr, cd.kind, cd.wm, cd.wn, cd.fm, cd.fn, cd.tk, cd.ks));
   TRY(launch_gemm(pr, pr->X.p));  // warm
   HIPCHK(hipEventRecord(e0, pr->stream));
@@ -729,17 +826,22 @@ static int set_all_live(mpci_prop *pr, int done) {
   return h2d(pr, pr->ctl.p, c.data(), sizeof(Ctl) * pr->P);
 }
 
-// Stage-side cost of a K split of ks: each split slot is one more partial
-// panel per direction that the GEMM writes and k_stage reads back, at the
-// stage kernel's streaming rate (about 5 TB/s measured).  Added to the
-// candidate's GEMM time when choosing, so that equal GEMM times favour the
-// smaller split.
-static double partial_cost_ms(const mpci_prop *pr, int ks) {
+// Stage-side cost of the partial layout set up for a candidate (pr->desc):
+// each slot is one more partial panel that k_stage reads back (its writes
+// are inside the GEMM's own timed launch).  The panels were written just
+// before, so the reads run from the device's caches: He 4 x 4000 at
+// P = 16, 13 more slots (40 MB) cost the stage kernel 4.5 us, ~9 TB/s.
+// Added to the candidate's GEMM time when choosing, so that equal GEMM
+// times favour fewer slots.
+static double partial_cost_ms(const mpci_prop *pr) {
   const mpci_system *sys = pr->sys;
+  const SysDesc &d = pr->desc;
   double bytes = 0.0;
   for (int L = 0; L + 1 < sys->nb; ++L)
-    bytes += (double)(sys->s[L] + sys->s[L + 1]) * 2.0 * pr->P * 8.0;
-  return 2.0 * ks * bytes / 5e12 * 1e3;
+    bytes += ((double)d.nrow_part[L] * sys->s[L] +
+              (double)d.ncol_part[L] * sys->s[L + 1]) *
+             2.0 * pr->P * 8.0;
+  return bytes / 9e12 * 1e3;
 }
 
 // Times every candidate of kGemmCands (or only `only`, when given), sets up
@@ -759,11 +861,11 @@ static int tune_gemm(mpci_prop *pr, const GemmCand *only) {
     } else {
       const bool lds_ok = gemm_lds_fits(pr);
       for (const GemmCand &cd : kGemmCands) {
-        if (cd.kind == 2 && !lds_ok) continue;
+        if (cd.kind >= 2 && !lds_ok) continue;
         float ms = 0.f;
         rc = time_gemm_cand(pr, cd, e0, e1, &ms);
         if (rc != MPCI_OK) break;
-        const double cost = ms + partial_cost_ms(pr, cd.ks);
+        const double cost = ms + partial_cost_ms(pr);
         if (cost < best_cost) {
           best_cost = cost;
           best = ms;
@@ -776,7 +878,7 @@ static int tune_gemm(mpci_prop *pr, const GemmCand *only) {
   (void)hipEventDestroy(e1);
   TRY(rc);
   TRY(set_all_live(pr, 1));
-  TRY(init_desc(pr, bc.ks));
+  TRY(init_desc_kind(pr, bc.kind, bc.ks));
   TRY(build_gemm_jobs(pr, bc.kind, bc.wm, bc.wn, bc.fm, bc.fn, bc.tk, bc.ks));
   pr->gemm_ms = best;
   if (!only)
@@ -792,7 +894,7 @@ static int autotune_gemm(mpci_prop *pr) {
   auto it = pr->sys->tuned.find(pr->P);
   if (it != pr->sys->tuned.end()) {
     const auto &t = it->second;
-    TRY(init_desc(pr, t.ks));
+    TRY(init_desc_kind(pr, t.kind, t.ks));
     TRY(build_gemm_jobs(pr, t.kind, t.wm, t.wn, t.fm, t.fn, t.tk, t.ks));
     pr->gemm_ms = t.ms;
     return MPCI_OK;
@@ -1682,6 +1784,9 @@ extern "C" int mpci_multi_adaptive_grouped(int n, mpci_prop *const *props,
     gg.part[i] = g.part;
     gg.d[i] = g.d;
     gg.ctl[i] = pr->ctl.p;
+    // a member tuned to k_gemm_pair keeps its own launch (its sums run in
+    // another order than the direct-to-LDS kernel's)
+    if (pr->gm_kind == 3) continue;
     jl[i] = gemm_job_list(pr, wm, wn, 2, 2, pr->gm_ks, i);
     njobs += jl[i].size();
   }
@@ -1727,8 +1832,20 @@ extern "C" int mpci_multi_adaptive_grouped(int n, mpci_prop *const *props,
   int rc = MPCI_OK;
   HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   for (int s = 0; s < kStages && rc == MPCI_OK; ++s) {
-    rc = launch_gemm_group(st, wm, wn, (unsigned)jobs.size(), djobs.p, gg, s);
+    if (!jobs.empty())
+      rc = launch_gemm_group(st, wm, wn, (unsigned)jobs.size(), djobs.p, gg, s);
     if (rc != MPCI_OK) break;
+    for (int i = 0; i < n; ++i) {
+      mpci_prop *pr = props[i];
+      if (pr->gm_kind != 3) continue;
+      hipLaunchKernelGGL(k_gemm_pair, dim3((unsigned)pr->gjobs.size()),
+                         dim3(512), 0, st, pr->dgjobs.p, pr->sys->Dr.p,
+                         pr->sys->dr_off.p,
+                         reinterpret_cast<const double *>(s == 0 ? pr->X.p
+                                                                 : pr->Xs.p),
+                         reinterpret_cast<double *>(pr->part.p), pr->ddesc.p,
+                         pr->ctl.p);
+    }
     if (s == kStages - 1)
       hipLaunchKernelGGL(k_stage_g<true>, dim3(wg_stage), dim3(256), 0, st,
                          dtab.p, n, s);
@@ -2279,7 +2396,7 @@ extern "C" int mpci_prop_set_gemm_candidate(mpci_prop *pr, int index) {
     return fail(MPCI_ERR_ARG, "GEMM candidate index out of range");
   if (pr->P < 2 || pr->sys->nb < 2)
     return fail(MPCI_ERR_ARG, "this propagator has no batched coupling");
-  if (kGemmCands[index].kind == 2 && !gemm_lds_fits(pr))
+  if (kGemmCands[index].kind >= 2 && !gemm_lds_fits(pr))
     return fail(MPCI_ERR_ARG, "candidate needs buffers below 2 GiB");
   HIPCHK(hipSetDevice(pr->sys->device));
   // the captured step refers to the job list and partial buffers, which

@@ -943,6 +943,209 @@ MPCI_GEMM_LDS(2, 4)
 MPCI_GEMM_LDS(4, 4)
 #undef MPCI_GEMM_LDS
 
+// ---------------------------------------------------------------------
+// Both coupling products of one dipole block from a single read of it
+// (P2 real columns of X in panels of 32).  Workgroup = 8 waves over a strip
+// of kPrR = 256 rows of D_L and a range [k0, k1) of its columns; wave w
+// owns strip rows 32 w .. 32 w + 31.  The strip is staged through LDS 16
+// columns at a time (direct-to-LDS loads, double-buffered), and each chunk
+// feeds two sets of v_mfma_f64_16x16x4f64:
+//  - D x_{L+1}: the wave's 32 rows x 32 columns, A = the chunk (rows by
+//    k), B = the chunk's 16 rows of X_{L+1} (staged beside it), held over
+//    the whole column range: one partial slot per column range (jb.q);
+//  - D^T x_L: the chunk's 16 columns x 32, A = the chunk transposed, B =
+//    the wave's 32 rows of X_L (loaded into registers once), summed over
+//    the 8 waves through LDS in a fixed order and stored per chunk: one
+//    partial slot per strip (jb.qt).
+// Chunk element (r, c) sits at r 16 + 2 ((c >> 1) ^ (r & 7)) + (c & 1):
+// the pair swizzle keeps both fragment reads (16 rows at one column, and
+// 16 columns of one row) on distinct banks.  Each output sums its K range
+// in increasing k, four per MFMA, like the other batched kernels, but the
+// D^T x_L sum over a strip is split over 8 waves first, so this kernel's
+// bits differ from theirs.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
+void k_gemm_pair(const GemmJob *__restrict__ jobs,
+                 const double *__restrict__ Dr,
+                 const long long *__restrict__ dr_off,
+                 const double *__restrict__ X, double *__restrict__ part,
+                 const SysDesc *__restrict__ dp, const Ctl *__restrict__ ctl) {
+  constexpr int R = kPrR, TK = kPrTK, TN = kPrTN;
+  constexpr int RS = 33;  // row stride of the cross-wave sums (doubles)
+  constexpr unsigned kOut = 0x80000000u;  // beyond any buffer range
+  // two of each buffer as separate arrays, so the compiler sees that the
+  // next chunk's loads never write what the current chunk reads
+  __shared__ __align__(16) double As0[R * TK], As1[R * TK];
+  __shared__ __align__(16) double Bs0[TK * TN], Bs1[TK * TN];
+  __shared__ __align__(16) double Rs0[8 * TK * RS], Rs1[8 * TK * RS];
+  __shared__ int live;
+  const GemmJob jb = jobs[blockIdx.x];
+  const SysDesc &d = *dp;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int P2 = 2 * d.P;
+  // skip the panel when every pulse it covers has finished
+  if (t == 0) live = 0;
+  __syncthreads();
+  {
+    const int p0 = jb.n0 >> 1, p1 = min(d.P, (jb.n0 + TN + 1) >> 1);
+    for (int p = p0 + t; p < p1; p += 512)
+      if (!ctl[p].done) live = 1;
+  }
+  __syncthreads();
+  if (!live) return;
+
+  const int L = jb.L;
+  const int sL = d.off[L + 1] - d.off[L];       // rows of D_L
+  const int sL1 = d.off[L + 2] - d.off[L + 1];  // columns of D_L
+  const int ld = jb.ld, r0 = jb.m0, n0 = jb.n0, c0 = jb.k0, c1 = jb.k1;
+  const __amdgpu_buffer_rsrc_t rsA = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(Dr + dr_off[L]), (short)0, (int)((long long)sL * ld * 8),
+      0x00020000);
+  const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(X + (long long)d.off[L + 1] * P2), (short)0,
+      (int)((long long)sL1 * P2 * 8), 0x00020000);
+
+  // chunk loads.  A: LDS pair (4 w + u) 64 + lane = (row r, slot s) holds
+  // columns 2 (s ^ (r & 7)) + {0, 1}; B (waves 0-3): pair w 64 + lane =
+  // (k, n) with k = pair >> 4, n = 2 (pair & 15), at k TN + n.
+  unsigned a_off[4];
+  int a_col[4];
+  bool a_row[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int pr = (4 * w + u) * 64 + lane;
+    const int r = pr >> 3, col = 2 * ((pr & 7) ^ (r & 7));
+    a_row[u] = r0 + r < sL;
+    a_col[u] = col;
+    a_off[u] = a_row[u] ? (unsigned)(((long long)(r0 + r) * ld + c0 + col) * 8)
+                        : 0u;
+  }
+  const int bk = (w * 64 + lane) >> 4, bn = 2 * (lane & 15);
+  const bool b_in = w < 4 && n0 + bn < P2;
+  const unsigned b_off =
+      b_in ? (unsigned)(((long long)(c0 + bk) * P2 + n0 + bn) * 8) : 0u;
+  auto issue = [&](int c, double *As, double *Bs) {
+    const int cc0 = c0 + c * TK;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool ok = a_row[u] && cc0 + a_col[u] < c1;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rsA, (lds_void_t *)(As + (4 * w + u) * 128), 16,
+          ok ? a_off[u] + (unsigned)(c * TK * 8) : kOut, 0, 0, 0);
+    }
+    if (w < 4) {
+      const bool ok = b_in && cc0 + bk < c1;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rsB, (lds_void_t *)(Bs + w * 128), 16,
+          ok ? b_off + (unsigned)(c * TK * P2 * 8) : kOut, 0, 0, 0);
+    }
+  };
+
+  // B of the transposed product: X_L over the wave's rows, lane (li, lk)
+  // of k-step kk holding row 32 w + 4 kk + lk, column 16 nt + li
+  const double *XL = X + (long long)d.off[L] * P2;
+  double xl[8][2];
+#pragma unroll
+  for (int kk = 0; kk < 8; ++kk)
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+      const int row = r0 + 32 * w + 4 * kk + lk, col = n0 + 16 * nt + li;
+      xl[kk][nt] =
+          (row < sL && col < P2) ? XL[(long long)row * P2 + col] : 0.0;
+    }
+
+  dmf4 accF[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) accF[a][b] = dmf4{0.0, 0.0, 0.0, 0.0};
+
+  double *outT = part + 2 * d.col_part_off[L] + (long long)jb.qt * sL1 * P2;
+  // the D^T x_L block of chunk c: the 8 waves' shares in Rs, summed in a
+  // fixed order, one output per thread
+  auto reduce = [&](int c, const double *Rs) {
+    const int m = t >> 5, n = t & 31;
+    double v = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < 8; ++ww) v += Rs[ww * TK * RS + m * RS + n];
+    const int col = c0 + c * TK + m;
+    if (col < c1 && n0 + n < P2) outT[(long long)col * P2 + n0 + n] = v;
+  };
+  // chunk c from A, B (landed) into the cross-wave buffer Rs; the previous
+  // chunk's block (in Rp, complete since the last barrier) is summed while
+  // this chunk's MFMAs run
+  auto step = [&](int c, const double *A, const double *B, double *Rs,
+                  const double *Rp) {
+    // D x_{L+1}: row 32 w + 16 bi + li, column 4 ks + lk of the chunk
+#pragma unroll
+    for (int ks = 0; ks < TK / 4; ++ks) {
+      double fa[2], fb[2];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi) {
+        const int rr = 32 * w + 16 * bi + li;
+        fa[bi] = A[rr * TK + 2 * ((2 * ks + (lk >> 1)) ^ (li & 7)) + (lk & 1)];
+      }
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt)
+        fb[nt] = B[(4 * ks + lk) * TN + 16 * nt + li];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+          accF[bi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              fa[bi], fb[nt], accF[bi][nt], 0, 0, 0);
+    }
+    // D^T x_L: chunk column li, strip row 32 w + 4 kk + lk
+    dmf4 accT[2] = {dmf4{0.0, 0.0, 0.0, 0.0}, dmf4{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      const int row = 32 * w + 4 * kk + lk;
+      const double ta = A[row * TK + 2 * ((li >> 1) ^ (row & 7)) + (li & 1)];
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt)
+        accT[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, xl[kk][nt],
+                                                        accT[nt], 0, 0, 0);
+    }
+    if (c > 0) reduce(c - 1, Rp);
+    // this wave's share of the chunk's 16 x 32 block of D^T x_L
+    double *Rw = Rs + w * TK * RS;
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        Rw[(4 * r + lk) * RS + 16 * nt + li] = accT[nt][r];
+    // every MFMA and LDS access of the chunk is issued before the wave
+    // waits at the barrier for the next chunk's loads
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();  // next chunk landed; Rs complete; A, B and Rp free
+  };
+  const int nch = (c1 - c0 + TK - 1) / TK;
+  if (nch > 0) issue(0, As0, Bs0);
+  __syncthreads();  // chunk 0 landed
+  int c = 0;
+  for (; c + 1 < nch; c += 2) {
+    issue(c + 1, As1, Bs1);
+    step(c, As0, Bs0, Rs0, Rs1);
+    if (c + 2 < nch) issue(c + 2, As0, Bs0);
+    step(c + 1, As1, Bs1, Rs1, Rs0);
+  }
+  if (c < nch) step(c, As0, Bs0, Rs0, Rs1);  // odd chunk count: the last
+  if (nch > 0) reduce(nch - 1, (nch - 1) & 1 ? Rs1 : Rs0);
+
+  double *outF = part + 2 * d.row_part_off[L] + (long long)jb.q * sL * P2;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = r0 + 32 * w + 16 * bi + 4 * r + lk;
+        const int n = n0 + 16 * nt + li;
+        if (row < sL && n < P2) outF[(long long)row * P2 + n] = accF[bi][nt][r];
+      }
+}
+
 #define MPCI_GEMM_DB(WM, WN, TK, FM, FN)                                      \
   template __global__ void k_gemm_db<WM, WN, TK, FM, FN>(                    \
       const GemmJob *, const double *, const long long *, const double *,    \

@@ -96,6 +96,9 @@ __global__ void k_gemm_lds(const GemmJob *jobs, const double *Dr,
                            double *part, const SysDesc *d, const Ctl *ctl);
 template <int WM, int WN>
 __global__ void k_gemm_lds_g(const GemmJob *jobs, GroupGemm g, int s);
+__global__ void k_gemm_pair(const GemmJob *jobs, const double *Dr,
+                            const long long *dr_off, const double *X,
+                            double *part, const SysDesc *d, const Ctl *ctl);
 template <int WM, int WN, int TK, int FM, int FN>
 __global__ void k_gemm_db(const GemmJob *jobs, const double *Dr,
                           const long long *dr_off, const double *X,

@@ -253,8 +253,8 @@ class Propagator:
         check(_lib.tdse_lib().mpci_prop_gemm_config(
             self.h, C.byref(wm), C.byref(wn), C.byref(ks), C.byref(ms),
             C.byref(kind), C.byref(tk), C.byref(fm), C.byref(fn)))
-        return {"kernel": ("k_gemm_coupling", "k_gemm_db",
-                           "k_gemm_lds")[kind.value],
+        return {"kernel": ("k_gemm_coupling", "k_gemm_db", "k_gemm_lds",
+                           "k_gemm_pair")[kind.value],
                 "wm": wm.value, "wn": wn.value, "fm": fm.value,
                 "fn": fn.value, "tk": tk.value, "ksplit": ks.value,
                 "ms": ms.value}

@@ -139,6 +139,36 @@ def test_grouped_equals_separate_runs(mpci, members):
                 (i, p)
 
 
+def test_grouped_with_pair_kernel_member(mpci, members):
+    """A member tuned to k_gemm_pair (both coupling directions from one read
+    of each dipole block) keeps its own coupling launch inside the grouped
+    graph, the other member joining the grouped job list: both still equal
+    their own separate runs bit for bit."""
+    ms = members[:2]
+
+    def pinned(m, i):
+        prop = fresh(mpci, m)
+        if i == 0:
+            for k in range(mpci.Propagator.gemm_candidates()):
+                cfg = prop.set_gemm_candidate(k)
+                if cfg["kernel"] == "k_gemm_pair" and cfg["ksplit"] == 1:
+                    break
+            assert prop.gemm_config()["kernel"] == "k_gemm_pair"
+        return prop
+
+    props = [pinned(m, i) for i, m in enumerate(ms)]
+    st = mpci.multi_adaptive(props, [0.0] * 2, [m["t1"] for m in ms],
+                             [0.05] * 2, TOL, TOL, grouped=True)
+    for i, m in enumerate(ms):
+        one = pinned(m, i)
+        s1, _ = one.prop_adaptive(0.0, m["t1"], 0.05, TOL, TOL)
+        for p in range(m["P"]):
+            assert st[i][p].accepted == s1[p].accepted, (i, p)
+            assert st[i][p].rejected == s1[p].rejected, (i, p)
+            assert np.array_equal(props[i].get_state(p), one.get_state(p)), \
+                (i, p)
+
+
 def test_grouped_refuses_single_pulse_member(mpci, members):
     props = [fresh(mpci, m) for m in members]  # the third has P = 1
     with pytest.raises(mpci.MpciError) as ei:

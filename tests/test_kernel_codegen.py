@@ -53,7 +53,7 @@ def test_no_scratch(device_asm):
 def test_expected_kernels_present(device_asm):
     for k in ("k_gemv_tile", "k_stage", "k_stage1", "k_finalize", "k_apply",
               "k_adj_stage", "k_adj_init", "k_dot_partial",
-              "k_gemm_coupling", "k_copy_strided"):
+              "k_gemm_coupling", "k_gemm_pair", "k_copy_strided"):
         assert k in device_asm, k
 
 
