@@ -276,7 +276,9 @@ int mpci_prop_kernel_time(mpci_prop *prop, double *total_ms,
  * mpci_prop_create -- wm x wn waves per workgroup, each wave fm x fn blocks
  * of 16 x 16 (a panel of 16 fm wm rows by 16 fn wn real columns) -- its K
  * split, kernel (0: single LDS buffer, 1: double-buffered with 16-byte
- * loads through registers, 2: double-buffered with direct-to-LDS loads) and
+ * loads through registers, 2: double-buffered with direct-to-LDS loads,
+ * 3: both coupling directions from one read of each dipole block, 8 waves,
+ * ks then being the workgroups per compute unit its split aims at) and
  * K chunk, and its measured time per coupling evaluation in ms.
  * Any pointer may be NULL. */
 int mpci_prop_gemm_config(mpci_prop *prop, int *wm, int *wn, int *ks,
