@@ -279,7 +279,8 @@ int mpci_prop_kernel_time(mpci_prop *prop, double *total_ms,
  * loads through registers, 2: double-buffered with direct-to-LDS loads,
  * 3: both coupling directions from one read of each dipole block, 8 waves,
  * ks then being the workgroups per compute unit its split aims at) and
- * K chunk, and its measured time per coupling evaluation in ms.
+ * K chunk, and its measured time per coupling evaluation in ms (the GEMM
+ * and the stage kernel that reads its partial sums back).
  * Any pointer may be NULL. */
 int mpci_prop_gemm_config(mpci_prop *prop, int *wm, int *wn, int *ks,
                           double *ms, int *kind, int *tk, int *fm, int *fn);

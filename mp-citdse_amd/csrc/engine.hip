@@ -803,19 +803,37 @@ static bool gemm_lds_fits(const mpci_prop *pr) {
   return true;
 }
 
-// Sets up candidate cd and times it: one warm evaluation, then three timed
-// between HIP events on the propagator's stream (all pulses live).
+static int launch_stage(mpci_prop *pr, const StageArgs &a);
+static StageArgs stage_args(mpci_prop *pr, int s);
+
+// Sets up candidate cd and times one coupling evaluation as a step runs
+// it: the GEMM and then the first stage kernel, which reads the
+// candidate's partial slots back (their number differs from candidate to
+// candidate).  Between HIP events on the propagator's stream, all pulses
+// live; the stage kernel writes only k_0 and the stage input buffer, so
+// the state is untouched.
 static int time_gemm_cand(mpci_prop *pr, const GemmCand &cd, hipEvent_t e0,
                           hipEvent_t e1, float *ms) {
   TRY(init_desc_kind(pr, cd.kind, cd.ks));
   TRY(build_gemm_jobs(pr, cd.kind, cd.wm, cd.wn, cd.fm, cd.fn, cd.tk, cd.ks));
-  TRY(launch_gemm(pr, pr->X.p));  // warm
+  const StageArgs a = stage_args(pr, 0);
+  // two warm evaluations, then eight timed: with fewer, candidates 5 %
+  // apart (k_gemm_pair against the best K-split tile at 16 pulses) came
+  // out in either order from one device to the next
+  constexpr int kReps = 8;
+  for (int r = 0; r < 2; ++r) {
+    TRY(launch_gemm(pr, pr->X.p));
+    TRY(launch_stage(pr, a));
+  }
   HIPCHK(hipEventRecord(e0, pr->stream));
-  for (int r = 0; r < 3; ++r) TRY(launch_gemm(pr, pr->X.p));
+  for (int r = 0; r < kReps; ++r) {
+    TRY(launch_gemm(pr, pr->X.p));
+    TRY(launch_stage(pr, a));
+  }
   HIPCHK(hipEventRecord(e1, pr->stream));
   HIPCHK(hipEventSynchronize(e1));
   HIPCHK(hipEventElapsedTime(ms, e0, e1));
-  *ms /= 3.0f;
+  *ms /= (float)kReps;
   return MPCI_OK;
 }
 
@@ -826,34 +844,16 @@ static int set_all_live(mpci_prop *pr, int done) {
   return h2d(pr, pr->ctl.p, c.data(), sizeof(Ctl) * pr->P);
 }
 
-// Stage-side cost of the partial layout set up for a candidate (pr->desc):
-// each slot is one more partial panel that k_stage reads back (its writes
-// are inside the GEMM's own timed launch).  The panels were written just
-// before, so the reads run from the device's caches: He 4 x 4000 at
-// P = 16, 13 more slots (40 MB) cost the stage kernel 4.5 us, ~9 TB/s.
-// Added to the candidate's GEMM time when choosing, so that equal GEMM
-// times favour fewer slots.
-static double partial_cost_ms(const mpci_prop *pr) {
-  const mpci_system *sys = pr->sys;
-  const SysDesc &d = pr->desc;
-  double bytes = 0.0;
-  for (int L = 0; L + 1 < sys->nb; ++L)
-    bytes += ((double)d.nrow_part[L] * sys->s[L] +
-              (double)d.ncol_part[L] * sys->s[L + 1]) *
-             2.0 * pr->P * 8.0;
-  return bytes / 9e12 * 1e3;
-}
-
 // Times every candidate of kGemmCands (or only `only`, when given), sets up
-// the one with the least GEMM time plus partial_cost_ms and, for a full
-// search, records it in the system's tune cache for this pulse count.
+// the fastest (GEMM plus the stage kernel's reading of its partial sums)
+// and, for a full search, records it in the system's tune cache for this
+// pulse count.
 static int tune_gemm(mpci_prop *pr, const GemmCand *only) {
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   int rc = set_all_live(pr, 0);
   float best = 1e30f;
-  double best_cost = 1e30;
   GemmCand bc = only ? *only : kGemmCands[0];
   if (rc == MPCI_OK) {
     if (only) {
@@ -865,9 +865,7 @@ static int tune_gemm(mpci_prop *pr, const GemmCand *only) {
         float ms = 0.f;
         rc = time_gemm_cand(pr, cd, e0, e1, &ms);
         if (rc != MPCI_OK) break;
-        const double cost = ms + partial_cost_ms(pr);
-        if (cost < best_cost) {
-          best_cost = cost;
+        if (ms < best) {
           best = ms;
           bc = cd;
         }
