@@ -44,12 +44,21 @@ TOL = 1e-10
 
 
 GRAPH = True
+PIN_GEMM = None  # "kernel:WMxWN:kKS" pins that autotuner candidate
 
 
 def make_prop(mpci, sysg, P=1):
     prop = mpci.Propagator(sysg, P=P)
     if not GRAPH:
         prop.set_graph(False)
+    if PIN_GEMM and P > 1:
+        for i in range(mpci.Propagator.gemm_candidates()):
+            cfg = prop.set_gemm_candidate(i)
+            if "%s:%dx%d:k%d" % (cfg["kernel"], cfg["wm"], cfg["wn"],
+                                 cfg["ksplit"]) == PIN_GEMM:
+                break
+        else:
+            raise SystemExit("--pin-gemm %s: no such candidate" % PIN_GEMM)
     return prop
 
 
@@ -569,14 +578,19 @@ def main():
                     help="after the timed run, launch a plain copy kernel "
                          "of 1 GiB read + 1 GiB write (scale reference for "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--pin-gemm", default=None, metavar="KERNEL:WMxWN:kKS",
+                    help="batched legs: pin this candidate of the GEMM "
+                         "autotuner (e.g. k_gemm_pair:8x1:k1) instead of its "
+                         "choice, for profiling one kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this implementation runs on a "
               "single MI355X (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    global GRAPH
+    global GRAPH, PIN_GEMM
     GRAPH = not args.no_graph
+    PIN_GEMM = args.pin_gemm
     import mpci
     from mpci import synth
     log("bench: building He system (4 x 4000) ...")
