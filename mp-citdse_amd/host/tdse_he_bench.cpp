@@ -1,10 +1,11 @@
 // Stand-alone driver of the single-pulse hot path over the C ABI, for
 // hardware-counter runs that need a plain process (no Python runtime).
 //
-//   tdse_he_bench [attempts] [copy_mib] [batch_pulses]
+//   tdse_he_bench [attempts] [copy_mib] [batch_pulses] [he|h]
 //
 // Builds the seeded synthetic He system (4 blocks of 4000 states, velocity
-// gauge), propagates one sin^2 pulse (w = 0.7, E0 = 0.03, 10 cycles) from the
+// gauge; with `h`, the H-shaped system of 21 blocks of 998 states that the
+// multi-instance bench leg uses), propagates one sin^2 pulse (w = 0.7, E0 = 0.03, 10 cycles) from the
 // ground state with the adaptive RKF78 (atol = rtol = 1e-10) for `attempts`
 // attempts, launched individually (no graph), then copies copy_mib MiB
 // device to device three times.  The copy reads and writes exactly
@@ -38,8 +39,12 @@ int main(int argc, char **argv) {
   const int attempts = argc > 1 ? std::atoi(argv[1]) : 20;
   const long long copy_mib = argc > 2 ? std::atoll(argv[2]) : 1024;
   const int batch = argc > 3 ? std::atoi(argv[3]) : 0;
-  const int nb = 4;
-  const int s[nb] = {4000, 4000, 4000, 4000};
+  const bool hshape = argc > 4 && argv[4][0] == 'h' && argv[4][1] == 0;
+  // the seeds and energy ranges of mpci/synth.py as bench.py uses them
+  const int nb = hshape ? 21 : 4;
+  const std::vector<int> s(nb, hshape ? 998 : 4000);
+  const unsigned long long seed = hshape ? 20261203ULL : 20261202ULL;
+  const double e_lo = hshape ? -0.5 : -2.9;
   std::vector<std::vector<double>> E(nb), D(nb - 1);
   std::vector<double *> Ep(nb), Dp(nb - 1);
   for (int L = 0; L < nb; ++L) {
@@ -50,11 +55,11 @@ int main(int argc, char **argv) {
     D[L].resize((size_t)s[L] * s[L + 1]);
     Dp[L] = D[L].data();
   }
-  if (mpci_synth_system(20261202ULL, nb, s, -2.9, 5.0, Ep.data(), Dp.data()))
+  if (mpci_synth_system(seed, nb, s.data(), e_lo, 5.0, Ep.data(), Dp.data()))
     return 1;
   std::vector<const double *> Ec(Ep.begin(), Ep.end()), Dc(Dp.begin(), Dp.end());
   mpci_system *sys = nullptr;
-  check(mpci_system_create(nb, s, 'v', Ec.data(), Dc.data(), &sys),
+  check(mpci_system_create(nb, s.data(), 'v', Ec.data(), Dc.data(), &sys),
         "mpci_system_create");
   mpci_prop *prop = nullptr;
   check(mpci_prop_create(sys, 1, &prop), "mpci_prop_create");
@@ -63,7 +68,7 @@ int main(int argc, char **argv) {
   const double w = 0.7, e0 = 0.03;
   check(mpci_pulse_sine(e0 * e0, w, 0.0, 10, 'v', &pl), "mpci_pulse_sine");
   check(mpci_prop_set_pulse(prop, 0, &pl), "mpci_prop_set_pulse");
-  const int N = 4 * 4000;
+  const int N = nb * s[0];
   std::vector<double> x(2 * (size_t)N, 0.0);
   x[0] = 1.0;
   check(mpci_prop_set_state(prop, 0, x.data()), "mpci_prop_set_state");
