@@ -100,8 +100,11 @@ def test_adjoint_vs_finite_differences(mpci, oracle, gauge, objective):
     assert rel(grad[0], fdo) < 1e-6
 
 
-def test_batched_adjoint_uneven_pulses(mpci, oracle):
-    """Four envelopes with different lengths in one batched adjoint."""
+@pytest.mark.parametrize("pin", [None, "k_gemm_pair"])
+def test_batched_adjoint_uneven_pulses(mpci, oracle, pin):
+    """Four envelopes with different lengths in one batched adjoint (with
+    the autotuned coupling, and with k_gemm_pair pinned: its forward and
+    transposed products both run in the adjoint's recompute and sweep)."""
     from mpci import synth
     E, g, o = setup(mpci, oracle, [32, 40, 28, 20], "v", seed=17)
     P = 4
@@ -112,6 +115,12 @@ def test_batched_adjoint_uneven_pulses(mpci, oracle):
     W = np.zeros(g.N)
     W[0] = 1.0
     prop = mpci.Propagator(g, P=P)
+    if pin:
+        for i in range(mpci.Propagator.gemm_candidates()):
+            cfg = prop.set_gemm_candidate(i)
+            if cfg["kernel"] == pin and cfg["ksplit"] == 1:
+                break
+        assert prop.gemm_config()["kernel"] == pin
     for p in range(P):
         prop.set_state(x0, p)
         prop.set_pulse(mpci.Pulse.pwl(envs[p], w, Ts[p]), p)
