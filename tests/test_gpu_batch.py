@@ -290,3 +290,29 @@ def test_full_he_shape_batch64(mpci, oracle):
         assert np.abs(one.get_state() - prop.get_state(p)).max() < 1e-11, p
         one.close()
     prop.close()
+
+
+@pytest.mark.parametrize("ks", [1, 2])
+def test_full_he_shape_pair_kernel(mpci, oracle, ks):
+    """k_gemm_pair pinned at the adjoint's batch size on the real He shape
+    (4 blocks of 4000 states: 16 strips of 256 rows per pair, the last one
+    160 rows, and 5 or 10 column ranges): its RHS against the oracle's on a
+    sample of pulses, at one and at two 32-column panels of X."""
+    g, o = build(mpci, oracle, [4000] * 4, "v", seed=20261202)
+    for P in (16, 20):
+        prop = mpci.Propagator(g, P=P)
+        for i in range(mpci.Propagator.gemm_candidates()):
+            cfg = prop.set_gemm_candidate(i)
+            if cfg["kernel"] == "k_gemm_pair" and cfg["ksplit"] == ks:
+                break
+        assert prop.gemm_config()["kernel"] == "k_gemm_pair"
+        rng = np.random.default_rng(P + ks)
+        x = rng.standard_normal((P, 2 * g.N))
+        F = rng.uniform(-0.05, 0.05, P)
+        got = prop.rhs(F, x.reshape(-1)).reshape(P, -1)
+        for p in (0, 7, P - 1):
+            ref = o.rhs(F[p], x[p])
+            assert np.abs(got[p] - ref).max() <= \
+                1e-13 * max(1, np.abs(ref).max()), (P, p)
+        prop.close()
+
