@@ -807,16 +807,18 @@ static int launch_stage(mpci_prop *pr, const StageArgs &a);
 static StageArgs stage_args(mpci_prop *pr, int s);
 
 // Sets up candidate cd and times one coupling evaluation as a step runs
-// it: the GEMM and then the first stage kernel, which reads the
-// candidate's partial slots back (their number differs from candidate to
-// candidate).  Between HIP events on the propagator's stream, all pulses
-// live; the stage kernel writes only k_0 and the stage input buffer, so
-// the state is untouched.
+// it: the GEMM and then a stage kernel, which reads the candidate's
+// partial slots back (their number differs from candidate to candidate).
+// The stage is a middle one (s = 6, three k_j in its combination), as
+// heavy as the average stage; stage 0 reads no k_j, and timing it made
+// the partial slots weigh more than they do in a step.  Between HIP
+// events on the propagator's stream, all pulses live; the stage kernel
+// writes only k_6 and the stage input buffer, so the state is untouched.
 static int time_gemm_cand(mpci_prop *pr, const GemmCand &cd, hipEvent_t e0,
                           hipEvent_t e1, float *ms) {
   TRY(init_desc_kind(pr, cd.kind, cd.ks));
   TRY(build_gemm_jobs(pr, cd.kind, cd.wm, cd.wn, cd.fm, cd.fn, cd.tk, cd.ks));
-  const StageArgs a = stage_args(pr, 0);
+  const StageArgs a = stage_args(pr, 6);
   // two warm evaluations, then eight timed: with fewer, candidates 5 %
   // apart (k_gemm_pair against the best K-split tile at 16 pulses) came
   // out in either order from one device to the next
