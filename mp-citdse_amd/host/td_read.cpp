@@ -62,7 +62,7 @@ int readConfig(std::string file, std::string &pot, std::string set_base,
        cycles = doc.as_int(field, "cycles"));
 
   if (ext) {
-    // optional keys of this build, absent from the reference's inputs
+    // optional extension keys, absent from the reference's inputs
     if (doc.has(prop, "mode")) ext->mode = doc.str(prop, "mode");
     if (doc.has(prop, "tol")) ext->atol = ext->rtol = doc.as_double(prop, "tol");
     if (doc.has(prop, "atol")) ext->atol = doc.as_double(prop, "atol");
