@@ -899,7 +899,14 @@ static int autotune_gemm(mpci_prop *pr) {
     pr->gemm_ms = t.ms;
     return MPCI_OK;
   }
-  return tune_gemm(pr, nullptr);
+  // timed on a state of nonzero components (the bit pattern 0x3FF00000 in
+  // every 32-bit word: 1.0000002), then cleared again
+  HIPCHK(hipMemsetD32Async(pr->X.p, (int)0x3FF00000u, (size_t)pr->NP * 4,
+                           pr->stream));
+  const int rc = tune_gemm(pr, nullptr);
+  HIPCHK(hipMemsetAsync(pr->X.p, 0, sizeof(double2) * pr->NP, pr->stream));
+  HIPCHK(hipStreamSynchronize(pr->stream));
+  return rc;
 }
 
 // Dipole bytes (8 S) up to which one pulse takes the small-system path by
