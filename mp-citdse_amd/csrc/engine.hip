@@ -818,7 +818,8 @@ static int time_gemm_cand(mpci_prop *pr, const GemmCand &cd, hipEvent_t e0,
                           hipEvent_t e1, float *ms) {
   TRY(init_desc_kind(pr, cd.kind, cd.ks));
   TRY(build_gemm_jobs(pr, cd.kind, cd.wm, cd.wn, cd.fm, cd.fn, cd.tk, cd.ks));
-  const StageArgs a = stage_args(pr, 6);
+  StageArgs a = stage_args(pr, 6);
+  a.xs_store = nullptr;  // leave any adjoint stage store alone
   // two warm evaluations, then eight timed: with fewer, candidates 5 %
   // apart (k_gemm_pair against the best K-split tile at 16 pulses) came
   // out in either order from one device to the next
