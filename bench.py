@@ -580,7 +580,7 @@ def main():
                          "the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--pin-gemm", default=None, metavar="KERNEL:WMxWN:kKS",
                     help="batched legs: pin this candidate of the GEMM "
-                         "autotuner (e.g. k_gemm_pair:8x1:k1) instead of its "
+                         "autotuner (e.g. k_gemm_pair:16x1:k1) instead of its "
                          "choice, for profiling one kernel")
     args = ap.parse_args()
     if args.gpus != 1:

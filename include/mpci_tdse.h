@@ -277,7 +277,7 @@ int mpci_prop_kernel_time(mpci_prop *prop, double *total_ms,
  * of 16 x 16 (a panel of 16 fm wm rows by 16 fn wn real columns) -- its K
  * split, kernel (0: single LDS buffer, 1: double-buffered with 16-byte
  * loads through registers, 2: double-buffered with direct-to-LDS loads,
- * 3: both coupling directions from one read of each dipole block, 8 waves,
+ * 3: both coupling directions from one read of each dipole block, 16 waves,
  * ks then being the workgroups per compute unit its split aims at) and
  * K chunk, and its measured time per coupling evaluation in ms (the GEMM
  * and the stage kernel that reads its partial sums back).

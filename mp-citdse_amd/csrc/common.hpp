@@ -62,7 +62,7 @@ struct GemmJob {
                 // is then the D x_{L+1} slot (column-range index)
 };
 
-// k_gemm_pair geometry: a workgroup of 8 waves takes a strip of kPrR rows
+// k_gemm_pair geometry: a workgroup of 16 waves takes a strip of kPrR rows
 // of D_L over a range of its columns, 16 columns (one LDS chunk) at a time,
 // for one 32-column panel of X.
 constexpr int kPrR = 256;

@@ -689,7 +689,7 @@ static int launch_gemm(mpci_prop *pr, const double2 *xin,
   const int wm = pr->gm_wm, wn = pr->gm_wn, tk = pr->gm_tk;
   const int f = pr->gm_fm * 10 + pr->gm_fn;
   if (pr->gm_kind == 3) {
-    hipExtLaunchKernelGGL(k_gemm_pair, grid, dim3(512), 0, pr->stream, e0, e1,
+    hipExtLaunchKernelGGL(k_gemm_pair, grid, dim3(1024), 0, pr->stream, e0, e1,
                           0, pr->dgjobs.p, sys->Dr.p, sys->dr_off.p, X, part,
                           pr->ddesc.p, pr->ctl.p);
   } else if (pr->gm_kind == 2) {
@@ -763,7 +763,7 @@ static int launch_gemm(mpci_prop *pr, const double2 *xin,
 // (double-buffered, 16-byte loads, K chunks of tk, fm x fn blocks per wave);
 // kind 2: k_gemm_lds (double-buffered direct-to-LDS loads, K chunks of 16,
 // 2 x 2 blocks per wave); kind 3: k_gemm_pair (both directions from one
-// read of each dipole block, 8 waves; ks is the number of workgroups per
+// read of each dipole block, 16 waves; ks is the number of workgroups per
 // compute unit its split aims at).
 struct GemmCand {
   int kind, wm, wn, fm, fn, tk, ks;
@@ -790,7 +790,7 @@ static const GemmCand kGemmCands[] = {
     {2, 2, 2, 2, 2, 16, 4}, {2, 4, 2, 2, 2, 16, 1}, {2, 4, 2, 2, 2, 16, 2},
     {2, 4, 2, 2, 2, 16, 4}, {2, 2, 4, 2, 2, 16, 1}, {2, 2, 4, 2, 2, 16, 2},
     {2, 2, 4, 2, 2, 16, 4}, {2, 4, 4, 2, 2, 16, 1}, {2, 4, 4, 2, 2, 16, 2},
-    {2, 4, 4, 2, 2, 16, 4}, {3, 8, 1, 2, 2, 16, 1}, {3, 8, 1, 2, 2, 16, 2}};
+    {2, 4, 4, 2, 2, 16, 4}, {3, 16, 1, 2, 2, 16, 1}, {3, 16, 1, 2, 2, 16, 2}};
 
 // k_gemm_lds addresses its operands through 32-bit buffer offsets: every
 // D_L copy and the X slab of the propagator must stay below 2 GiB.
@@ -806,33 +806,31 @@ static bool gemm_lds_fits(const mpci_prop *pr) {
 static int launch_stage(mpci_prop *pr, const StageArgs &a);
 static StageArgs stage_args(mpci_prop *pr, int s);
 
-// Sets up candidate cd and times one coupling evaluation as a step runs
-// it: the GEMM and then a stage kernel, which reads the candidate's
-// partial slots back (their number differs from candidate to candidate).
-// The stage is a middle one (s = 6, three k_j in its combination), as
-// heavy as the average stage; stage 0 reads no k_j, and timing it made
-// the partial slots weigh more than they do in a step.  Between HIP
-// events on the propagator's stream, all pulses live; the stage kernel
-// writes only k_6 and the stage input buffer, so the state is untouched.
+// Sets up candidate cd and times the coupling evaluations of a step as the
+// step runs them: stages 0 .. 11 in order, each the GEMM (on the state at
+// stage 0, on the stage input the previous stage kernel wrote after that)
+// and then the stage kernel, which reads the candidate's partial slots
+// back (their number differs from candidate to candidate).  Between HIP
+// events on the propagator's stream, all pulses live; the stage kernels
+// write only k_s and the stage input buffer, so the state is untouched.
+// (Timing one stage over and over, on the state, ranked k_gemm_pair
+// behind the best K-split tile at 16 pulses on most devices, while whole
+// attempts run it 2-5 % faster.)
 static int time_gemm_cand(mpci_prop *pr, const GemmCand &cd, hipEvent_t e0,
                           hipEvent_t e1, float *ms) {
   TRY(init_desc_kind(pr, cd.kind, cd.ks));
   TRY(build_gemm_jobs(pr, cd.kind, cd.wm, cd.wn, cd.fm, cd.fn, cd.tk, cd.ks));
-  StageArgs a = stage_args(pr, 6);
-  a.xs_store = nullptr;  // leave any adjoint stage store alone
-  // two warm evaluations, then eight timed: with fewer, candidates 5 %
-  // apart (k_gemm_pair against the best K-split tile at 16 pulses) came
-  // out in either order from one device to the next
-  constexpr int kReps = 8;
-  for (int r = 0; r < 2; ++r) {
-    TRY(launch_gemm(pr, pr->X.p));
-    TRY(launch_stage(pr, a));
-  }
+  auto eval = [&](int s) {
+    TRY(launch_gemm(pr, s == 0 ? pr->X.p : pr->Xs.p));
+    StageArgs a = stage_args(pr, s);
+    a.xs_store = nullptr;  // leave any adjoint stage store alone
+    return launch_stage(pr, a);
+  };
+  // two warm evaluations, then the twelve stages timed
+  constexpr int kReps = kStages - 1;
+  for (int s = 0; s < 2; ++s) TRY(eval(s));
   HIPCHK(hipEventRecord(e0, pr->stream));
-  for (int r = 0; r < kReps; ++r) {
-    TRY(launch_gemm(pr, pr->X.p));
-    TRY(launch_stage(pr, a));
-  }
+  for (int s = 0; s < kReps; ++s) TRY(eval(s));
   HIPCHK(hipEventRecord(e1, pr->stream));
   HIPCHK(hipEventSynchronize(e1));
   HIPCHK(hipEventElapsedTime(ms, e0, e1));
@@ -1847,7 +1845,7 @@ extern "C" int mpci_multi_adaptive_grouped(int n, mpci_prop *const *props,
       mpci_prop *pr = props[i];
       if (pr->gm_kind != 3) continue;
       hipLaunchKernelGGL(k_gemm_pair, dim3((unsigned)pr->gjobs.size()),
-                         dim3(512), 0, st, pr->dgjobs.p, pr->sys->Dr.p,
+                         dim3(1024), 0, st, pr->dgjobs.p, pr->sys->Dr.p,
                          pr->sys->dr_off.p,
                          reinterpret_cast<const double *>(s == 0 ? pr->X.p
                                                                  : pr->Xs.p),

@@ -945,26 +945,31 @@ MPCI_GEMM_LDS(4, 4)
 
 // ---------------------------------------------------------------------
 // Both coupling products of one dipole block from a single read of it
-// (P2 real columns of X in panels of 32).  Workgroup = 8 waves over a strip
-// of kPrR = 256 rows of D_L and a range [k0, k1) of its columns; wave w
-// owns strip rows 32 w .. 32 w + 31.  The strip is staged through LDS 16
-// columns at a time (direct-to-LDS loads, double-buffered), and each chunk
-// feeds two sets of v_mfma_f64_16x16x4f64:
-//  - D x_{L+1}: the wave's 32 rows x 32 columns, A = the chunk (rows by
-//    k), B = the chunk's 16 rows of X_{L+1} (staged beside it), held over
-//    the whole column range: one partial slot per column range (jb.q);
-//  - D^T x_L: the chunk's 16 columns x 32, A = the chunk transposed, B =
-//    the wave's 32 rows of X_L (loaded into registers once), summed over
-//    the 8 waves through LDS in a fixed order and stored per chunk: one
+// (P2 real columns of X in panels of 32).  Workgroup = 16 waves over a
+// strip of kPrR = 256 rows of D_L and a range [k0, k1) of its columns.
+// The strip is staged through LDS 16 columns at a time (direct-to-LDS
+// loads, double-buffered), and each chunk feeds two sets of
+// v_mfma_f64_16x16x4f64, one per half of the workgroup, wave vw of each
+// half owning strip rows 32 vw .. 32 vw + 31:
+//  - waves 0-7, D x_{L+1}: the wave's 32 rows x 32 columns, A = the chunk
+//    (rows by k), B = the chunk's 16 rows of X_{L+1} (staged beside it),
+//    held over the whole column range: one partial slot per column range
+//    (jb.q);
+//  - waves 8-15, D^T x_L: the chunk's 16 columns x 32, A = the chunk
+//    transposed, B = the wave's 32 rows of X_L (loaded into registers
+//    once), summed over the 8 waves through LDS in a fixed order (by waves
+//    0-7, while the next chunk's MFMAs run) and stored per chunk: one
 //    partial slot per strip (jb.qt).
-// Chunk element (r, c) sits at r 16 + 2 ((c >> 1) ^ (r & 7)) + (c & 1):
-// the pair swizzle keeps both fragment reads (16 rows at one column, and
-// 16 columns of one row) on distinct banks.  Each output sums its K range
-// in increasing k, four per MFMA, like the other batched kernels, but the
-// D^T x_L sum over a strip is split over 8 waves first, so this kernel's
-// bits differ from theirs.
+// Splitting the roles over 16 waves (four per SIMD) rather than giving each
+// of 8 waves both measured 123.3 against 125.5 us per evaluation (He,
+// P = 16).  Chunk element (r, c) sits at r 16 + 2 ((c >> 1) ^ (r & 7)) +
+// (c & 1): the pair swizzle keeps both fragment reads (16 rows at one
+// column, and 16 columns of one row) on distinct banks.  Each output sums
+// its K range in increasing k, four per MFMA, like the other batched
+// kernels, but the D^T x_L sum over a strip is split over 8 waves first,
+// so this kernel's bits differ from theirs.
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
 void k_gemm_pair(const GemmJob *__restrict__ jobs,
                  const double *__restrict__ Dr,
                  const long long *__restrict__ dr_off,
@@ -983,13 +988,15 @@ void k_gemm_pair(const GemmJob *__restrict__ jobs,
   const SysDesc &d = *dp;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int li = lane & 15, lk = lane >> 4;
+  const bool fwd = w < 8;  // waves 0-7: D x_{L+1}; waves 8-15: D^T x_L
+  const int vw = w & 7;    // the wave's strip rows 32 vw .. 32 vw + 31
   const int P2 = 2 * d.P;
   // skip the panel when every pulse it covers has finished
   if (t == 0) live = 0;
   __syncthreads();
   {
     const int p0 = jb.n0 >> 1, p1 = min(d.P, (jb.n0 + TN + 1) >> 1);
-    for (int p = p0 + t; p < p1; p += 512)
+    for (int p = p0 + t; p < p1; p += 1024)
       if (!ctl[p].done) live = 1;
   }
   __syncthreads();
@@ -1006,15 +1013,15 @@ void k_gemm_pair(const GemmJob *__restrict__ jobs,
       (void *)(X + (long long)d.off[L + 1] * P2), (short)0,
       (int)((long long)sL1 * P2 * 8), 0x00020000);
 
-  // chunk loads.  A: LDS pair (4 w + u) 64 + lane = (row r, slot s) holds
+  // chunk loads.  A: LDS pair (2 w + u) 64 + lane = (row r, slot s) holds
   // columns 2 (s ^ (r & 7)) + {0, 1}; B (waves 0-3): pair w 64 + lane =
   // (k, n) with k = pair >> 4, n = 2 (pair & 15), at k TN + n.
-  unsigned a_off[4];
-  int a_col[4];
-  bool a_row[4];
+  unsigned a_off[2];
+  int a_col[2];
+  bool a_row[2];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int pr = (4 * w + u) * 64 + lane;
+  for (int u = 0; u < 2; ++u) {
+    const int pr = (2 * w + u) * 64 + lane;
     const int r = pr >> 3, col = 2 * ((pr & 7) ^ (r & 7));
     a_row[u] = r0 + r < sL;
     a_col[u] = col;
@@ -1028,10 +1035,10 @@ void k_gemm_pair(const GemmJob *__restrict__ jobs,
   auto issue = [&](int c, double *As, double *Bs) {
     const int cc0 = c0 + c * TK;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 2; ++u) {
       const bool ok = a_row[u] && cc0 + a_col[u] < c1;
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
-          rsA, (lds_void_t *)(As + (4 * w + u) * 128), 16,
+          rsA, (lds_void_t *)(As + (2 * w + u) * 128), 16,
           ok ? a_off[u] + (unsigned)(c * TK * 8) : kOut, 0, 0, 0);
     }
     if (w < 4) {
@@ -1042,79 +1049,81 @@ void k_gemm_pair(const GemmJob *__restrict__ jobs,
     }
   };
 
-  // B of the transposed product: X_L over the wave's rows, lane (li, lk)
-  // of k-step kk holding row 32 w + 4 kk + lk, column 16 nt + li
+  // one register block per wave, by role: the D x_{L+1} accumulators
+  // (bi, nt) in reg[2 bi + nt], or the X_L fragments (kk, nt: row
+  // 32 vw + 4 kk + lk, column 16 nt + li) in element 2 kk + nt of reg
   const double *XL = X + (long long)d.off[L] * P2;
-  double xl[8][2];
+  dmf4 reg[4];
 #pragma unroll
-  for (int kk = 0; kk < 8; ++kk)
+  for (int a = 0; a < 4; ++a) reg[a] = dmf4{0.0, 0.0, 0.0, 0.0};
+  if (!fwd) {
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt) {
-      const int row = r0 + 32 * w + 4 * kk + lk, col = n0 + 16 * nt + li;
-      xl[kk][nt] =
-          (row < sL && col < P2) ? XL[(long long)row * P2 + col] : 0.0;
-    }
-
-  dmf4 accF[2][2];
+    for (int kk = 0; kk < 8; ++kk)
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) accF[a][b] = dmf4{0.0, 0.0, 0.0, 0.0};
+      for (int nt = 0; nt < 2; ++nt) {
+        const int row = r0 + 32 * vw + 4 * kk + lk, col = n0 + 16 * nt + li;
+        reg[(2 * kk + nt) >> 2][(2 * kk + nt) & 3] =
+            (row < sL && col < P2) ? XL[(long long)row * P2 + col] : 0.0;
+      }
+  }
 
   double *outT = part + 2 * d.col_part_off[L] + (long long)jb.qt * sL1 * P2;
-  // the D^T x_L block of chunk c: the 8 waves' shares in Rs, summed in a
-  // fixed order, one output per thread
+  // the D^T x_L block of chunk c: the 8 transposed waves' shares in Rs,
+  // summed in a fixed order, one output per thread of waves 0-7
   auto reduce = [&](int c, const double *Rs) {
-    const int m = t >> 5, n = t & 31;
-    double v = 0.0;
+    if (t < 512) {
+      const int m = t >> 5, n = t & 31;
+      double v = 0.0;
 #pragma unroll
-    for (int ww = 0; ww < 8; ++ww) v += Rs[ww * TK * RS + m * RS + n];
-    const int col = c0 + c * TK + m;
-    if (col < c1 && n0 + n < P2) outT[(long long)col * P2 + n0 + n] = v;
+      for (int ww = 0; ww < 8; ++ww) v += Rs[ww * TK * RS + m * RS + n];
+      const int col = c0 + c * TK + m;
+      if (col < c1 && n0 + n < P2) outT[(long long)col * P2 + n0 + n] = v;
+    }
   };
-  // chunk c from A, B (landed) into the cross-wave buffer Rs; the previous
-  // chunk's block (in Rp, complete since the last barrier) is summed while
-  // this chunk's MFMAs run
+  // chunk c from A, B (landed); the transposed waves leave their shares in
+  // Rs, the forward waves sum the previous chunk's block (in Rp, complete
+  // since the last barrier) behind their MFMAs
   auto step = [&](int c, const double *A, const double *B, double *Rs,
                   const double *Rp) {
-    // D x_{L+1}: row 32 w + 16 bi + li, column 4 ks + lk of the chunk
+    if (fwd) {
 #pragma unroll
-    for (int ks = 0; ks < TK / 4; ++ks) {
-      double fa[2], fb[2];
+      for (int ks = 0; ks < TK / 4; ++ks) {
+        double fa[2], fb[2];
 #pragma unroll
-      for (int bi = 0; bi < 2; ++bi) {
-        const int rr = 32 * w + 16 * bi + li;
-        fa[bi] = A[rr * TK + 2 * ((2 * ks + (lk >> 1)) ^ (li & 7)) + (lk & 1)];
-      }
-#pragma unroll
-      for (int nt = 0; nt < 2; ++nt)
-        fb[nt] = B[(4 * ks + lk) * TN + 16 * nt + li];
-#pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
+        for (int bi = 0; bi < 2; ++bi) {
+          const int rr = 32 * vw + 16 * bi + li;
+          fa[bi] = A[rr * TK + 2 * ((2 * ks + (lk >> 1)) ^ (li & 7)) + (lk & 1)];
+        }
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt)
-          accF[bi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-              fa[bi], fb[nt], accF[bi][nt], 0, 0, 0);
-    }
-    // D^T x_L: chunk column li, strip row 32 w + 4 kk + lk
-    dmf4 accT[2] = {dmf4{0.0, 0.0, 0.0, 0.0}, dmf4{0.0, 0.0, 0.0, 0.0}};
+          fb[nt] = B[(4 * ks + lk) * TN + 16 * nt + li];
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-      const int row = 32 * w + 4 * kk + lk;
-      const double ta = A[row * TK + 2 * ((li >> 1) ^ (row & 7)) + (li & 1)];
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+          for (int nt = 0; nt < 2; ++nt)
+            reg[2 * bi + nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                fa[bi], fb[nt], reg[2 * bi + nt], 0, 0, 0);
+      }
+      if (c > 0) reduce(c - 1, Rp);
+    } else {
+      dmf4 accT[2] = {dmf4{0.0, 0.0, 0.0, 0.0}, dmf4{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk) {
+        const int row = 32 * vw + 4 * kk + lk;
+        const double ta = A[row * TK + 2 * ((li >> 1) ^ (row & 7)) + (li & 1)];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+          accT[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              ta, reg[(2 * kk + nt) >> 2][(2 * kk + nt) & 3], accT[nt], 0, 0,
+              0);
+      }
+      double *Rw = Rs + vw * TK * RS;
 #pragma unroll
       for (int nt = 0; nt < 2; ++nt)
-        accT[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, xl[kk][nt],
-                                                        accT[nt], 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          Rw[(4 * r + lk) * RS + 16 * nt + li] = accT[nt][r];
     }
-    if (c > 0) reduce(c - 1, Rp);
-    // this wave's share of the chunk's 16 x 32 block of D^T x_L
-    double *Rw = Rs + w * TK * RS;
-#pragma unroll
-    for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        Rw[(4 * r + lk) * RS + 16 * nt + li] = accT[nt][r];
     // every MFMA and LDS access of the chunk is issued before the wave
     // waits at the barrier for the next chunk's loads
     __builtin_amdgcn_sched_barrier(0);
@@ -1133,17 +1142,20 @@ void k_gemm_pair(const GemmJob *__restrict__ jobs,
   if (c < nch) step(c, As0, Bs0, Rs0, Rs1);  // odd chunk count: the last
   if (nch > 0) reduce(nch - 1, (nch - 1) & 1 ? Rs1 : Rs0);
 
-  double *outF = part + 2 * d.row_part_off[L] + (long long)jb.q * sL * P2;
+  if (fwd) {
+    double *outF = part + 2 * d.row_part_off[L] + (long long)jb.q * sL * P2;
 #pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
+    for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt)
+      for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = r0 + 32 * w + 16 * bi + 4 * r + lk;
-        const int n = n0 + 16 * nt + li;
-        if (row < sL && n < P2) outF[(long long)row * P2 + n] = accF[bi][nt][r];
-      }
+        for (int r = 0; r < 4; ++r) {
+          const int row = r0 + 32 * vw + 16 * bi + 4 * r + lk;
+          const int n = n0 + 16 * nt + li;
+          if (row < sL && n < P2)
+            outF[(long long)row * P2 + n] = reg[2 * bi + nt][r];
+        }
+  }
 }
 
 #define MPCI_GEMM_DB(WM, WN, TK, FM, FN)                                      \
